@@ -1,0 +1,114 @@
+/*
+ * brdc.h -- C ABI of the MI355X boundary-row (BR) values-only
+ * divide-and-conquer eigensolver for symmetric tridiagonal matrices.
+ *
+ * Arithmetic is FP64 throughout.  D holds the diagonal (n entries), E the
+ * off-diagonal (n-1 entries); on success D is overwritten by the eigenvalues
+ * in ascending order.  Indices are 0-based; sizes are element counts unless
+ * named *_bytes.  Plain pointers and sizes only: any FFI (ctypes, cgo, JNI,
+ * Fortran ISO_C_BINDING) binds these symbols directly.
+ */
+#ifndef BRDC_H
+#define BRDC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status of the device entry points (the launch itself; numerical failures
+ * are reported through the device-side info word).  Mirrors the reference's
+ * evd_status (gpu/include/evd/status.h:8-18) for the codes this path uses. */
+typedef enum brdc_status {
+    BRDC_STATUS_SUCCESS = 0,
+    BRDC_STATUS_INVALID_VALUE = 3,
+    BRDC_STATUS_INVALID_POINTER = 2,
+    BRDC_STATUS_INVALID_WORKSPACE = 4,
+    BRDC_STATUS_ALLOC_FAILED = 5,
+    BRDC_STATUS_HIP_ERROR = 6,
+    BRDC_STATUS_INTERNAL_ERROR = 7
+} brdc_status;
+
+/* Tuning knobs of the device path; zero-initialise for the defaults. */
+typedef struct brdc_options {
+    int leaf_max;     /* largest leaf (rows) solved by the wavefront Jacobi
+                         kernel, 2..64; 0 selects the default (32) */
+    int reserved[7];  /* must be zero */
+} brdc_options;
+
+/*
+ * Host-array entry.  Replaces DSTEDC_DC_BR (cpu/src/dstedc_dc_br.f:7-8), the
+ * reference's values-only BR driver, with the same argument meaning:
+ *   n       order of the matrix (n >= 0)
+ *   d[n]    in: diagonal; out: eigenvalues ascending
+ *   e[n-1]  in: off-diagonal; contents are not used afterwards
+ *   work, lwork, iwork, liwork  workspace as in the reference; lwork == -1 or
+ *           liwork == -1 is a query that stores the minimum sizes in work[0]
+ *           and iwork[0] ((16 + 2*NTHR)*n and 7*n for n > 25, else 1 and 1,
+ *           with NTHR = 1: the solve itself runs on the GPU)
+ * Returns INFO: 0 success; -1 bad n; -5 lwork too small; -7 liwork too small;
+ * > 0 numerical failure of a tree node (the reference's encoding,
+ * dlaed0_br.f:302-303 remapped at dstedc_dc_br.f:123-124: the failing node's
+ * first row r and size m give r*(n+1) + r + m - 1, r 1-based) or non-finite
+ * input (n + 1); below -1000, a HIP runtime failure (-1000 - brdc_status).
+ * Blocks the calling thread; device memory is allocated for the call and
+ * released before it returns.
+ */
+int brdc_dstebr(int n, double *d, double *e, double *work, int lwork, int *iwork,
+                int liwork);
+
+/* Fortran-callable alias with all arguments by reference (LP64), the calling
+ * convention of dstedc_dc_br_ (scripts/cpu/smoke_br.f90:24-27). */
+void brdc_dstebr_(const int *n, double *d, double *e, double *work, const int *lwork,
+                  int *iwork, const int *liwork, int *info);
+
+/*
+ * Device workspace query.  Replaces the reference's
+ * evd::internal::tridiag_solver_workspace_size (gpu/src/tridiag_solver.cu:2373).
+ * Writes the bytes the device entry needs for order n (O(n): about 40 doubles
+ * per row; no O(n^2) slab).  Returns a brdc_status.
+ */
+int brdc_dstebr_device_workspace(int n, size_t *bytes);
+int brdc_dstebr_device_workspace_ex(int n, const brdc_options *opts, size_t *bytes);
+
+/*
+ * Device entry.  Replaces evd::internal::tridiag_solver_run_from_device_tridiagonal
+ * (gpu/src/tridiag_solver.cu:2541-2583) on the values-only path:
+ *   d_dev[n]     device; in: diagonal, out: eigenvalues ascending
+ *   e_dev[n-1]   device; off-diagonal, not modified
+ *   workspace    device, caller-owned, >= brdc_dstebr_device_workspace bytes,
+ *                256-byte aligned
+ *   stream       hipStream_t the work is enqueued on (NULL = default stream)
+ *   info_dev     device int receiving INFO as brdc_dstebr defines it (written
+ *                by the last kernel of the solve; may be NULL)
+ * Asynchronous: returns after enqueueing, with no host synchronisation and no
+ * device allocation, so it can be captured into a hipGraph.  Returns a
+ * brdc_status for the launch.
+ */
+int brdc_dstebr_device(int n, double *d_dev, const double *e_dev, void *workspace,
+                       size_t workspace_bytes, void *stream, int *info_dev);
+int brdc_dstebr_device_ex(int n, double *d_dev, const double *e_dev, void *workspace,
+                          size_t workspace_bytes, void *stream, int *info_dev,
+                          const brdc_options *opts);
+
+/*
+ * Per-merge diagnostics of the last solve run in a workspace (the role of
+ * TridiagMergeLevelDiagnostics, gpu/src/handle_impl.hpp:16-85, and of the
+ * per-merge (N, K) record of the instrumented CPU copy).  Copies up to
+ * max_merges records of 6 ints -- level (0 = root merge), first row, size,
+ * K (non-deflated poles), G (close-pole Givens deflations), secular function
+ * evaluations summed over the merge's roots -- into out_host, ordered by
+ * level from the root down, and returns the number of merges (or a negative
+ * brdc_status).  Synchronises the stream.
+ */
+int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void *stream,
+                     const brdc_options *opts, int *out_host, int max_merges);
+
+/* Human-readable name of a brdc_status. */
+const char *brdc_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRDC_H */

@@ -1,0 +1,131 @@
+"""brdc -- MI355X boundary-row values-only D&C tridiagonal eigensolver.
+
+Host-side mirror of the reference's entry points for this path:
+
+* :func:`dstebr` -- DSTEDC_DC_BR semantics (cpu/src/dstedc_dc_br.f:7-151) on
+  host arrays: returns (eigenvalues ascending, INFO).
+* :func:`dstebr_device` -- the device entry (the role of
+  tridiag_solver_run_from_device_tridiagonal, gpu/src/tridiag_solver.cu:2541)
+  on torch tensors already resident in HBM, enqueued on the current stream.
+* :func:`workspace_bytes`, :func:`merge_stats` -- workspace query and per-merge
+  diagnostics.
+
+All compute runs in libbrdc.so (hand-written gfx950 HIP kernels); torch only
+provides device memory and the stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import BrdcError, options
+
+__all__ = ["dstebr", "dstebr_device", "workspace_bytes", "merge_stats", "Solver", "BrdcError",
+           "options", "query"]
+
+
+def query(n: int) -> tuple[int, int, int]:
+    """Workspace query of the host entry (LWORK = -1): (info, lwmin, liwmin)."""
+    L = _lib.load()
+    w = (ctypes.c_double * 1)()
+    iw = (ctypes.c_int * 1)()
+    info = L.brdc_dstebr(int(n), None, None, w, -1, iw, -1)
+    return info, int(w[0]), int(iw[0])
+
+
+def dstebr(d, e):
+    """Eigenvalues of the symmetric tridiagonal (d, e) via the host-array entry.
+
+    Returns (w, info) with w ascending; d and e are not modified.
+    """
+    L = _lib.load()
+    d = np.array(d, dtype=np.float64, copy=True)
+    n = d.shape[0]
+    e = np.ascontiguousarray(np.asarray(e, dtype=np.float64).reshape(-1)[: max(n - 1, 0)]).copy()
+    if e.shape[0] != max(n - 1, 0):
+        raise ValueError("e must hold n-1 entries")
+    info, lw, liw = query(n)
+    work = np.zeros(max(lw, 1))
+    iwork = np.zeros(max(liw, 1), dtype=np.int32)
+    e_buf = np.zeros(max(n, 1))
+    e_buf[: e.shape[0]] = e
+    info = L.brdc_dstebr(n, d.ctypes.data_as(_lib._dp), e_buf.ctypes.data_as(_lib._dp),
+                         work.ctypes.data_as(_lib._dp), lw, iwork.ctypes.data_as(_lib._ip), liw)
+    return d, int(info)
+
+
+def workspace_bytes(n: int, leaf_max: int = 0) -> int:
+    L = _lib.load()
+    out = ctypes.c_size_t(0)
+    o = options(leaf_max)
+    _lib.check(L.brdc_dstebr_device_workspace_ex(int(n), ctypes.byref(o), ctypes.byref(out)),
+               "workspace query")
+    return int(out.value)
+
+
+class Solver:
+    """A reusable device solve for one order n: owns the workspace and info word.
+
+    ``solve(d, e)`` enqueues on the current torch stream and overwrites the
+    device tensor ``d`` with the eigenvalues; nothing synchronises.
+    """
+
+    def __init__(self, n: int, device="cuda", leaf_max: int = 0):
+        import torch
+        self.n = int(n)
+        self.opts = options(leaf_max)
+        self.bytes = workspace_bytes(self.n, leaf_max)
+        self.ws = torch.empty(max(self.bytes, 256), dtype=torch.uint8, device=device)
+        self.info = torch.zeros(1, dtype=torch.int32, device=device)
+        if self.ws.data_ptr() % 256:
+            raise BrdcError("workspace allocation is not 256-byte aligned")
+
+    def solve(self, d, e, stream=None):
+        import torch
+        if d.dtype != torch.float64 or e.dtype != torch.float64:
+            raise TypeError("d and e must be float64")
+        if not (d.is_cuda and e.is_cuda and d.is_contiguous() and e.is_contiguous()):
+            raise ValueError("d and e must be contiguous device tensors")
+        if d.numel() != self.n or e.numel() < max(self.n - 1, 0):
+            raise ValueError("shape mismatch with the solver's n")
+        st = stream if stream is not None else torch.cuda.current_stream()
+        L = _lib.load()
+        _lib.check(L.brdc_dstebr_device_ex(self.n, d.data_ptr(), e.data_ptr() if self.n > 1 else None,
+                                           self.ws.data_ptr(), self.bytes, st.cuda_stream,
+                                           self.info.data_ptr(), ctypes.byref(self.opts)),
+                   "brdc_dstebr_device")
+        return d
+
+    def merge_stats(self, stream=None):
+        """Per-merge (level, start, size, K, G, secular evaluations) of the last solve."""
+        import torch
+        L = _lib.load()
+        st = stream if stream is not None else torch.cuda.current_stream()
+        total = max(1, self.n)
+        buf = np.zeros(6 * total, dtype=np.int32)
+        cnt = L.brdc_merge_stats(self.n, self.ws.data_ptr(), self.bytes, st.cuda_stream,
+                                 ctypes.byref(self.opts), buf.ctypes.data_as(_lib._ip), total)
+        if cnt < 0:
+            _lib.check(-cnt, "merge stats")
+        rec = buf[: 6 * cnt].reshape(cnt, 6)
+        out = np.zeros(cnt, dtype=[("level", "i4"), ("start", "i4"), ("size", "i4"),
+                                   ("k", "i4"), ("givens", "i4"), ("evals", "i4")])
+        for i, name in enumerate(out.dtype.names):
+            out[name] = rec[:, i]
+        return out
+
+
+def dstebr_device(d, e, leaf_max: int = 0):
+    """Eigenvalues of device tensors (d, e); returns (w, info) with w a new tensor."""
+    import torch
+    s = Solver(d.numel(), device=d.device, leaf_max=leaf_max)
+    w = d.clone()
+    s.solve(w, e.contiguous())
+    torch.cuda.synchronize(d.device)
+    return w, int(s.info.item())
+
+
+def merge_stats(solver: Solver):
+    return solver.merge_stats()

@@ -1,0 +1,80 @@
+"""ctypes binding of libbrdc.so (the C ABI declared in include/brdc.h).
+
+The library is built in place by ``make -C ppopp27-brdc-ae_amd`` (or
+``__graft_entry__.build()``).  Nothing here falls back to a CPU path: if the
+library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libbrdc.so")
+
+# brdc_status codes (include/brdc.h)
+STATUS = {0: "success", 2: "invalid pointer", 3: "invalid value", 4: "invalid workspace",
+          5: "alloc failed", 6: "hip error", 7: "internal error"}
+
+#: every symbol include/brdc.h declares
+EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
+           "brdc_dstebr_device_workspace_ex", "brdc_dstebr_device", "brdc_dstebr_device_ex",
+           "brdc_merge_stats", "brdc_status_string")
+
+
+class Options(ctypes.Structure):
+    """brdc_options: leaf_max (0 = default) plus reserved words."""
+    _fields_ = [("leaf_max", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class BrdcError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BrdcError(f"HIP library {LIB_PATH} is not built; run "
+                        f"`make -C {PKG_ROOT}` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.brdc_dstebr.restype = ctypes.c_int
+    L.brdc_dstebr.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, ctypes.c_int]
+    L.brdc_dstebr_.restype = None
+    L.brdc_dstebr_.argtypes = [_ip, _dp, _dp, _dp, _ip, _ip, _ip, _ip]
+    L.brdc_dstebr_device_workspace.restype = ctypes.c_int
+    L.brdc_dstebr_device_workspace.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+    L.brdc_dstebr_device_workspace_ex.restype = ctypes.c_int
+    L.brdc_dstebr_device_workspace_ex.argtypes = [ctypes.c_int, ctypes.POINTER(Options),
+                                                  ctypes.POINTER(ctypes.c_size_t)]
+    L.brdc_dstebr_device.restype = ctypes.c_int
+    L.brdc_dstebr_device.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp]
+    L.brdc_dstebr_device_ex.restype = ctypes.c_int
+    L.brdc_dstebr_device_ex.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp,
+                                        ctypes.POINTER(Options)]
+    L.brdc_merge_stats.restype = ctypes.c_int
+    L.brdc_merge_stats.argtypes = [ctypes.c_int, _vp, ctypes.c_size_t, _vp,
+                                   ctypes.POINTER(Options), _ip, ctypes.c_int]
+    L.brdc_status_string.restype = ctypes.c_char_p
+    L.brdc_status_string.argtypes = [ctypes.c_int]
+    _lib = L
+    return L
+
+
+def options(leaf_max: int = 0) -> Options:
+    o = Options()
+    o.leaf_max = int(leaf_max)
+    return o
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = load().brdc_status_string(status).decode()
+        raise BrdcError(f"{what}: {msg} (status {status})")

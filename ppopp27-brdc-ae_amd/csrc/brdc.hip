@@ -1,0 +1,355 @@
+// brdc.hip -- host driver and C ABI of the MI355X BR values-only D&C solver.
+//
+// One solve is a fixed, host-sync-free launch sequence on the caller's stream:
+//   k_norm, k_prep_e, k_leaf, then per tree level (deepest first)
+//   prepare -> deflate -> secular -> [wprod -> rows] -> finalize
+// (wprod/rows are skipped at the root merge, whose boundary rows nobody
+// reads -- WANTQ = .FALSE. at dlaed0_br.f:182).  The tree shape depends only
+// on n and the leaf size, so the sequence is fixed per n and can be captured
+// into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/brdc.h"
+#include "brdc_kernels.h"
+
+namespace brdc {
+namespace {
+
+constexpr int kDefaultLeaf = 32;
+constexpr size_t kAlign = 256;
+
+inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Workspace carve-up; identical for the query and the run.
+struct Layout {
+    size_t scal, ew;
+    size_t sd[2], slo[2], shi[2];
+    size_t md, mz, mq1, mq2;
+    size_t ad, az, aq1, aq2;
+    size_t gd, gq1, gq2;
+    size_t td, tq1, tq2;
+    size_t rlam, rtau, rn1, rn2;
+    size_t w, qo1, qo2;
+    size_t rorg, P, rend, fl;
+    size_t counts, info;
+    size_t total;
+    int levels;
+    int merges;  // 2^levels (merge records, index (2^k - 1) + i at depth k)
+};
+
+Layout make_layout(int n, int leaf) {
+    Layout L{};
+    const size_t nd = (size_t)std::max(n, 1) * sizeof(double);
+    const size_t ni = (size_t)std::max(n, 1) * sizeof(int);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = align_up(off + bytes);
+        return at;
+    };
+    L.levels = (n > 0) ? tree_levels(n, leaf) : 0;
+    L.merges = 1 << L.levels;
+    L.scal = take(4 * sizeof(double));
+    L.ew = take(nd);
+    for (int b = 0; b < 2; b++) {
+        L.sd[b] = take(nd);
+        L.slo[b] = take(nd);
+        L.shi[b] = take(nd);
+    }
+    L.md = take(nd);
+    L.mz = take(nd);
+    L.mq1 = take(nd);
+    L.mq2 = take(nd);
+    L.ad = take(nd);
+    L.az = take(nd);
+    L.aq1 = take(nd);
+    L.aq2 = take(nd);
+    L.gd = take(nd);
+    L.gq1 = take(nd);
+    L.gq2 = take(nd);
+    L.td = take(nd);
+    L.tq1 = take(nd);
+    L.tq2 = take(nd);
+    L.rlam = take(nd);
+    L.rtau = take(nd);
+    L.rn1 = take(nd);
+    L.rn2 = take(nd);
+    L.w = take(nd);
+    L.qo1 = take(nd);
+    L.qo2 = take(nd);
+    L.rorg = take(ni);
+    L.P = take(ni);
+    L.rend = take(ni);
+    L.fl = take(ni);
+    L.counts = take((size_t)4 * L.merges * sizeof(int));
+    L.info = take(sizeof(int));
+    L.total = off;
+    return L;
+}
+
+int leaf_of(const brdc_options* o) {
+    if (!o || o->leaf_max == 0) return kDefaultLeaf;
+    return o->leaf_max;
+}
+
+bool options_ok(const brdc_options* o) {
+    if (!o) return true;
+    if (o->leaf_max != 0 && (o->leaf_max < 2 || o->leaf_max > kMaxLeaf)) return false;
+    for (int r : o->reserved)
+        if (r != 0) return false;
+    return true;
+}
+
+template <class T>
+T* at(void* ws, size_t off) {
+    return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
+}
+
+#define BRDC_LAUNCH_CHECK()                                        \
+    do {                                                           \
+        if (hipGetLastError() != hipSuccess) return BRDC_STATUS_HIP_ERROR; \
+    } while (0)
+
+int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, hipStream_t st,
+                  int* info_user, int leaf) {
+    double* scal = at<double>(ws, L.scal);
+    double* ew = at<double>(ws, L.ew);
+    int* info = at<int>(ws, L.info);
+    int* counts_all = at<int>(ws, L.counts);
+
+    hipLaunchKernelGGL(k_norm, dim3(1), dim3(1024), 0, st, n, d, e, scal, info, info_user);
+    BRDC_LAUNCH_CHECK();
+    if (n <= 1) return BRDC_STATUS_SUCCESS;
+    hipLaunchKernelGGL(k_prep_e, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, d, e, scal, ew);
+    BRDC_LAUNCH_CHECK();
+
+    const int levels = L.levels;
+    int cur = 0;
+    double* sd[2] = {at<double>(ws, L.sd[0]), at<double>(ws, L.sd[1])};
+    double* slo[2] = {at<double>(ws, L.slo[0]), at<double>(ws, L.slo[1])};
+    double* shi[2] = {at<double>(ws, L.shi[0]), at<double>(ws, L.shi[1])};
+    const int single = (levels == 0);
+    hipLaunchKernelGGL(k_leaf, dim3(1 << levels), dim3(kWave), 0, st, n, levels, d, ew, scal,
+                       single ? d : sd[cur], slo[cur], shi[cur], info, single, info_user);
+    BRDC_LAUNCH_CHECK();
+    if (single) return BRDC_STATUS_SUCCESS;
+
+    double* md = at<double>(ws, L.md);
+    double* mz = at<double>(ws, L.mz);
+    double* mq1 = at<double>(ws, L.mq1);
+    double* mq2 = at<double>(ws, L.mq2);
+    double* ad = at<double>(ws, L.ad);
+    double* az = at<double>(ws, L.az);
+    double* aq1 = at<double>(ws, L.aq1);
+    double* aq2 = at<double>(ws, L.aq2);
+    double* gd = at<double>(ws, L.gd);
+    double* gq1 = at<double>(ws, L.gq1);
+    double* gq2 = at<double>(ws, L.gq2);
+    double* td = at<double>(ws, L.td);
+    double* tq1 = at<double>(ws, L.tq1);
+    double* tq2 = at<double>(ws, L.tq2);
+    double* rlam = at<double>(ws, L.rlam);
+    double* rtau = at<double>(ws, L.rtau);
+    double* rn1 = at<double>(ws, L.rn1);
+    double* rn2 = at<double>(ws, L.rn2);
+    double* w = at<double>(ws, L.w);
+    double* qo1 = at<double>(ws, L.qo1);
+    double* qo2 = at<double>(ws, L.qo2);
+    int* rorg = at<int>(ws, L.rorg);
+    int* P = at<int>(ws, L.P);
+    int* rend = at<int>(ws, L.rend);
+    int* fl = at<int>(ws, L.fl);
+
+    for (int depth = levels - 1; depth >= 0; depth--) {
+        const int merges = 1 << depth;
+        const int maxsz = max_node_size(n, depth);
+        const int bpm = (maxsz + kThreads - 1) / kThreads;
+        const dim3 grid(merges * bpm);
+        int* counts = counts_all + 4 * (merges - 1);
+        const int nxt = cur ^ 1;
+        const int root = (depth == 0);
+
+        hipLaunchKernelGGL(k_merge_prepare, grid, dim3(kThreads), 0, st, n, depth, bpm, sd[cur],
+                           slo[cur], shi[cur], ew, md, mz, mq1, mq2);
+        BRDC_LAUNCH_CHECK();
+        if (maxsz <= 512) {
+            hipLaunchKernelGGL(k_deflate<64>, dim3(merges), dim3(64), 0, st, n, depth, ew, md, mz,
+                               mq1, mq2, P, rend, fl, ad, az, aq1, aq2, gd, gq1, gq2, td, tq1,
+                               tq2, counts);
+        } else if (maxsz <= 8192) {
+            hipLaunchKernelGGL(k_deflate<256>, dim3(merges), dim3(256), 0, st, n, depth, ew, md,
+                               mz, mq1, mq2, P, rend, fl, ad, az, aq1, aq2, gd, gq1, gq2, td, tq1,
+                               tq2, counts);
+        } else {
+            hipLaunchKernelGGL(k_deflate<1024>, dim3(merges), dim3(1024), 0, st, n, depth, ew, md,
+                               mz, mq1, mq2, P, rend, fl, ad, az, aq1, aq2, gd, gq1, gq2, td, tq1,
+                               tq2, counts);
+        }
+        BRDC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_secular, grid, dim3(kThreads), 0, st, n, depth, bpm, ew, counts, ad,
+                           az, rlam, rtau, rn1, rn2, rorg, counts, info);
+        BRDC_LAUNCH_CHECK();
+        if (!root) {
+            hipLaunchKernelGGL(k_wprod, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad, az,
+                               rtau, rn1, rn2, rorg, w);
+            BRDC_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_rows, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad, aq1,
+                               aq2, w, rtau, rn1, rn2, rorg, qo1, qo2);
+            BRDC_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_finalize, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, rlam,
+                           qo1, qo2, gd, gq1, gq2, td, tq1, tq2, root ? d : sd[nxt], slo[nxt],
+                           shi[nxt], root, scal, info, info_user);
+        BRDC_LAUNCH_CHECK();
+        cur = nxt;
+    }
+    return BRDC_STATUS_SUCCESS;
+}
+
+}  // namespace
+}  // namespace brdc
+
+using namespace brdc;
+
+extern "C" {
+
+const char* brdc_status_string(int status) {
+    switch (status) {
+        case BRDC_STATUS_SUCCESS: return "success";
+        case BRDC_STATUS_INVALID_POINTER: return "invalid pointer";
+        case BRDC_STATUS_INVALID_VALUE: return "invalid value";
+        case BRDC_STATUS_INVALID_WORKSPACE: return "workspace too small or misaligned";
+        case BRDC_STATUS_ALLOC_FAILED: return "device allocation failed";
+        case BRDC_STATUS_HIP_ERROR: return "HIP runtime error";
+        case BRDC_STATUS_INTERNAL_ERROR: return "internal error";
+        default: return "unknown status";
+    }
+}
+
+int brdc_dstebr_device_workspace_ex(int n, const brdc_options* opts, size_t* bytes) {
+    if (!bytes) return BRDC_STATUS_INVALID_POINTER;
+    if (n < 0 || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
+    *bytes = make_layout(n, leaf_of(opts)).total;
+    return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_dstebr_device_workspace(int n, size_t* bytes) {
+    return brdc_dstebr_device_workspace_ex(n, nullptr, bytes);
+}
+
+int brdc_dstebr_device_ex(int n, double* d_dev, const double* e_dev, void* workspace,
+                          size_t workspace_bytes, void* stream, int* info_dev,
+                          const brdc_options* opts) {
+    if (n < 0 || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
+    if (n == 0) return BRDC_STATUS_SUCCESS;
+    if (!d_dev || (n > 1 && !e_dev) || !workspace) return BRDC_STATUS_INVALID_POINTER;
+    const int leaf = leaf_of(opts);
+    const Layout L = make_layout(n, leaf);
+    if (workspace_bytes < L.total || (reinterpret_cast<uintptr_t>(workspace) % kAlign) != 0)
+        return BRDC_STATUS_INVALID_WORKSPACE;
+    return enqueue_solve(n, d_dev, e_dev, workspace, L, static_cast<hipStream_t>(stream), info_dev,
+                         leaf);
+}
+
+int brdc_dstebr_device(int n, double* d_dev, const double* e_dev, void* workspace,
+                       size_t workspace_bytes, void* stream, int* info_dev) {
+    return brdc_dstebr_device_ex(n, d_dev, e_dev, workspace, workspace_bytes, stream, info_dev,
+                                 nullptr);
+}
+
+int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void* stream,
+                     const brdc_options* opts, int* out_host, int max_merges) {
+    if (n < 0 || !options_ok(opts)) return -BRDC_STATUS_INVALID_VALUE;
+    if (!workspace || (!out_host && max_merges > 0)) return -BRDC_STATUS_INVALID_POINTER;
+    const Layout L = make_layout(n, leaf_of(opts));
+    if (workspace_bytes < L.total) return -BRDC_STATUS_INVALID_WORKSPACE;
+    const int total = (n > 1) ? (1 << L.levels) - 1 : 0;
+    if (total == 0) return 0;
+    std::vector<int> counts((size_t)4 * total);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(counts.data(),
+                       static_cast<const char*>(workspace) + L.counts,
+                       counts.size() * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -BRDC_STATUS_HIP_ERROR;
+    int rec = 0;
+    for (int depth = 0; depth < L.levels; depth++) {
+        for (int i = 0; i < (1 << depth); i++) {
+            if (rec >= max_merges) return total;
+            int s, sz;
+            node_geom(n, depth, i, &s, &sz);
+            const int* c = &counts[(size_t)4 * ((1 << depth) - 1 + i)];
+            int* o = out_host + 6 * rec;
+            o[0] = depth;
+            o[1] = s;
+            o[2] = sz;
+            o[3] = c[0];
+            o[4] = c[1];
+            o[5] = c[3];
+            rec++;
+        }
+    }
+    return total;
+}
+
+int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork, int liwork) {
+    // argument checks and workspace query as dstedc_dc_br.f:44-75 (NTHR = 1)
+    const int smlsiz = 25;
+    const bool lquery = (lwork == -1 || liwork == -1);
+    if (n < 0) return -1;
+    const long lwmin = (n <= smlsiz) ? 1 : std::max<long>(1, 18L * n);
+    const long liwmin = (n <= smlsiz) ? 1 : std::max<long>(1, 7L * n);
+    if (work) work[0] = (double)lwmin;
+    if (iwork) iwork[0] = (int)liwmin;
+    if (!lquery) {
+        if (lwork < lwmin) return -5;
+        if (liwork < liwmin) return -7;
+    } else {
+        return 0;
+    }
+    if (n <= 1) return 0;
+
+    const Layout L = make_layout(n, kDefaultLeaf);
+    double *dd = nullptr, *de = nullptr;
+    void* ws = nullptr;
+    int* dinfo = nullptr;
+    int info = 0;
+    int result = 0;
+    if (hipMalloc(&dd, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&de, sizeof(double) * std::max(n - 1, 1)) != hipSuccess ||
+        hipMalloc(&ws, L.total) != hipSuccess || hipMalloc(&dinfo, sizeof(int)) != hipSuccess) {
+        result = -1000 - BRDC_STATUS_ALLOC_FAILED;
+    } else if (hipMemcpy(dd, d, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice) != hipSuccess) {
+        result = -1000 - BRDC_STATUS_HIP_ERROR;
+    } else {
+        const int st = enqueue_solve(n, dd, de, ws, L, nullptr, dinfo, kDefaultLeaf);
+        if (st != BRDC_STATUS_SUCCESS ||
+            hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            result = -1000 - BRDC_STATUS_HIP_ERROR;
+        } else {
+            result = info;
+        }
+    }
+    (void)hipFree(dd);
+    (void)hipFree(de);
+    (void)hipFree(ws);
+    (void)hipFree(dinfo);
+    if (work) work[0] = (double)lwmin;
+    if (iwork) iwork[0] = (int)liwmin;
+    return result;
+}
+
+void brdc_dstebr_(const int* n, double* d, double* e, double* work, const int* lwork, int* iwork,
+                  const int* liwork, int* info) {
+    *info = brdc_dstebr(*n, d, e, work, *lwork, iwork, *liwork);
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol that
+include/brdc.h declares, and its host-side logic (workspace queries, argument
+errors) behaves like DSTEDC_DC_BR's (cpu/src/dstedc_dc_br.f:44-75)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from brdc import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "brdc.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(brdc_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib.load()
+    names = header_functions()
+    assert set(names) == set(_lib.EXPORTS)
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_status_strings():
+    L = _lib.load()
+    assert L.brdc_status_string(0) == b"success"
+    assert L.brdc_status_string(4).startswith(b"workspace")
+
+
+def test_host_query_matches_reference_convention():
+    import brdc
+    assert brdc.query(4096) == (0, 18 * 4096, 7 * 4096)
+    assert brdc.query(25) == (0, 1, 1)
+    assert brdc.query(26) == (0, 18 * 26, 7 * 26)
+    assert brdc.query(-3)[0] == -1
+
+
+def test_host_argument_errors():
+    L = _lib.load()
+    d = (ctypes.c_double * 100)()
+    e = (ctypes.c_double * 100)()
+    w = (ctypes.c_double * 10)()
+    iw = (ctypes.c_int * 10000)()
+    assert L.brdc_dstebr(100, d, e, w, 10, iw, 10000) == -5
+    big = (ctypes.c_double * 2000)()
+    assert L.brdc_dstebr(100, d, e, big, 2000, iw, 10) == -7
+    # n <= 1 returns before any device work
+    assert L.brdc_dstebr(1, d, e, w, 1, iw, 1) == 0
+    assert L.brdc_dstebr(0, d, e, w, 1, iw, 1) == 0
+
+
+def test_device_workspace_query_is_linear_in_n():
+    import brdc
+    b1 = brdc.workspace_bytes(16384)
+    b2 = brdc.workspace_bytes(65536)
+    assert 0 < b1 < b2 < 5 * b1                     # O(n), no n^2 slab
+    assert b2 < 400 * 65536                         # < 400 bytes per row
+    L = _lib.load()
+    out = ctypes.c_size_t(0)
+    assert L.brdc_dstebr_device_workspace(-1, ctypes.byref(out)) == 3
+    o = _lib.options(65)
+    assert L.brdc_dstebr_device_workspace_ex(100, ctypes.byref(o), ctypes.byref(out)) == 3
+    o = _lib.options(64)
+    assert L.brdc_dstebr_device_workspace_ex(100, ctypes.byref(o), ctypes.byref(out)) == 0
+
+
+def test_device_entry_rejects_bad_arguments_without_launching():
+    L = _lib.load()
+    o = _lib.Options()
+    # null workspace / pointers and undersized workspace fail before any launch
+    assert L.brdc_dstebr_device(100, 0x1000, 0x2000, None, 0, None, None) == 2
+    assert L.brdc_dstebr_device(100, None, 0x2000, 0x1000, 1 << 30, None, None) == 2
+    assert L.brdc_dstebr_device(100, 0x1000, 0x2000, 0x100000, 16, None, None) == 4
+    assert L.brdc_dstebr_device(100, 0x1000, 0x2000, 0x100010, 1 << 30, None, None) == 4
+    assert L.brdc_dstebr_device(-2, 0x1000, 0x2000, 0x100000, 1 << 30, None, None) == 3
+    o.reserved[0] = 1
+    assert L.brdc_dstebr_device_ex(100, 0x1000, 0x2000, 0x100000, 1 << 30, None, None,
+                                   ctypes.byref(o)) == 3
+    assert L.brdc_dstebr_device(0, None, None, None, 0, None, None) == 0
