@@ -105,6 +105,19 @@ int brdc_dstebr_device_ex(int n, double *d_dev, const double *e_dev, void *works
 int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void *stream,
                      const brdc_options *opts, int *out_host, int max_merges);
 
+/*
+ * Kernel-class timing of the solves enqueued from the calling host thread
+ * between begin and end (diagnostics only; the role of the reference's
+ * MergeLevelTimer, gpu/src/tridiag_solver.cu:158-359).  Each launch is
+ * bracketed by two HIP events on its own stream.  end synchronises those
+ * events and writes, for each of the first nclasses classes in the order
+ * norm, prep_e, leaf, prepare, deflate, secular, wprod, rows, finalize
+ * (BRDC_KERNEL_CLASSES of them), the summed milliseconds and launch counts.
+ */
+#define BRDC_KERNEL_CLASSES 9
+int brdc_profile_begin(void);
+int brdc_profile_end(double *ms_out, int *launches_out, int nclasses);
+
 /* Human-readable name of a brdc_status. */
 const char *brdc_status_string(int status);
 

@@ -23,6 +23,7 @@ from . import _lib
 from ._lib import BrdcError, options
 
 __all__ = ["dstebr", "dstebr_device", "workspace_bytes", "merge_stats", "Solver", "BrdcError",
+           "profile_kernels",
            "options", "query"]
 
 
@@ -129,3 +130,56 @@ def dstebr_device(d, e, leaf_max: int = 0):
 
 def merge_stats(solver: Solver):
     return solver.merge_stats()
+
+
+#: algorithmic HBM bytes per matrix row per level for the memory-bound kernels
+#: (prepare: read d, blo, bhi + write d, z, q1, q2; deflate: read 4 + write 4
+#: doubles plus flags and indices; finalize: read 3 + write 3 doubles; leaf:
+#: read d, e + write d, blo, bhi)
+_BYTES_PER_ROW = {"prepare": 56, "deflate": 112, "finalize": 48, "leaf": 40}
+
+
+def profile_kernels(solver: Solver, problems, stream=None):
+    """Run one solve per (w, d, e) in problems with kernel-class timing on.
+
+    Returns {"kernels": {class: {"ms", "launches", "avg_us"}}, "flops": {class:
+    FP64 flops}, "bytes": {class: bytes}} summed over the problems, where the
+    flop counts use each solve's own per-merge K and secular evaluation
+    counts: 7 flops per pole per evaluation (secular), 3 per pole pair
+    (product-form w, non-root merges), 9 per pole pair (boundary rows,
+    non-root merges).
+    """
+    import torch
+    L = _lib.load()
+    nc = len(_lib.KERNEL_CLASSES)
+    tot_ms = np.zeros(nc)
+    tot_n = np.zeros(nc, dtype=np.int64)
+    flops = {"secular": 0.0, "wprod": 0.0, "rows": 0.0}
+    nbytes = {k: 0.0 for k in _BYTES_PER_ROW}
+    st = stream if stream is not None else torch.cuda.current_stream()
+    for (w, d, e) in problems:
+        w.copy_(d)
+        torch.cuda.synchronize()
+        _lib.check(L.brdc_profile_begin(), "profile begin")
+        solver.solve(w, e, st)
+        ms = np.zeros(nc)
+        cnt = np.zeros(nc, dtype=np.int32)
+        _lib.check(L.brdc_profile_end(ms.ctypes.data_as(_lib._dp), cnt.ctypes.data_as(_lib._ip), nc),
+                   "profile end")
+        tot_ms += ms
+        tot_n += cnt
+        stats = solver.merge_stats(st)
+        k = stats["k"].astype(np.float64)
+        flops["secular"] += float(np.sum(7.0 * k * stats["evals"]))
+        nonroot = (stats["level"] > 0) & (stats["k"] > 2)
+        flops["wprod"] += float(np.sum(3.0 * k[nonroot] ** 2))
+        flops["rows"] += float(np.sum(9.0 * k[nonroot] ** 2))
+        levels = int(stats["level"].max()) + 1 if len(stats) else 0
+        for kname, b in _BYTES_PER_ROW.items():
+            nbytes[kname] += b * solver.n * (1 if kname == "leaf" else levels)
+    kern = {}
+    for i, name in enumerate(_lib.KERNEL_CLASSES):
+        if tot_n[i]:
+            kern[name] = {"ms": float(tot_ms[i]), "launches": int(tot_n[i]),
+                          "avg_us": float(tot_ms[i] * 1e3 / tot_n[i])}
+    return {"kernels": kern, "flops": flops, "bytes": nbytes}

@@ -19,7 +19,10 @@ STATUS = {0: "success", 2: "invalid pointer", 3: "invalid value", 4: "invalid wo
 #: every symbol include/brdc.h declares
 EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
            "brdc_dstebr_device_workspace_ex", "brdc_dstebr_device", "brdc_dstebr_device_ex",
-           "brdc_merge_stats", "brdc_status_string")
+           "brdc_merge_stats", "brdc_status_string", "brdc_profile_begin", "brdc_profile_end")
+
+KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
+                  "finalize")
 
 
 class Options(ctypes.Structure):
@@ -62,6 +65,10 @@ def load() -> ctypes.CDLL:
     L.brdc_merge_stats.restype = ctypes.c_int
     L.brdc_merge_stats.argtypes = [ctypes.c_int, _vp, ctypes.c_size_t, _vp,
                                    ctypes.POINTER(Options), _ip, ctypes.c_int]
+    L.brdc_profile_begin.restype = ctypes.c_int
+    L.brdc_profile_begin.argtypes = []
+    L.brdc_profile_end.restype = ctypes.c_int
+    L.brdc_profile_end.argtypes = [_dp, _ip, ctypes.c_int]
     L.brdc_status_string.restype = ctypes.c_char_p
     L.brdc_status_string.argtypes = [ctypes.c_int]
     _lib = L

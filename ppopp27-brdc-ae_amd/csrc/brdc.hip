@@ -116,6 +116,44 @@ T* at(void* ws, size_t off) {
         if (hipGetLastError() != hipSuccess) return BRDC_STATUS_HIP_ERROR; \
     } while (0)
 
+// Kernel-class timing (diagnostics, the role of the reference's
+// MergeLevelTimer, gpu/src/tridiag_solver.cu:158-359): when enabled on the
+// calling host thread, every launch is bracketed by two events on its stream.
+enum KernelClass { kNorm, kPrepE, kLeaf, kPrepare, kDeflate, kSecular, kWprod, kRows, kFinalize,
+                   kNumClasses };
+struct Profiler {
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+};
+thread_local Profiler g_prof;
+
+struct ProfScope {
+    int cls;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(int c, hipStream_t s) : cls(c), st(s) {
+        if (!g_prof.on) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+        (void)hipEventRecord(a, st);
+    }
+    ~ProfScope() {
+        if (!g_prof.on || !a || !b) return;
+        (void)hipEventRecord(b, st);
+        g_prof.marks.push_back({cls, {a, b}});
+    }
+};
+
+// deflation block size follows the level's node size
+template <class... Args>
+void launch_deflate(int maxsz, int merges, hipStream_t st, Args... args) {
+    if (maxsz <= 512)
+        hipLaunchKernelGGL(k_deflate<64>, dim3(merges), dim3(64), 0, st, args...);
+    else if (maxsz <= 8192)
+        hipLaunchKernelGGL(k_deflate<256>, dim3(merges), dim3(256), 0, st, args...);
+    else
+        hipLaunchKernelGGL(k_deflate<1024>, dim3(merges), dim3(1024), 0, st, args...);
+}
+
 int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, hipStream_t st,
                   int* info_user, int leaf) {
     double* scal = at<double>(ws, L.scal);
@@ -123,10 +161,17 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     int* info = at<int>(ws, L.info);
     int* counts_all = at<int>(ws, L.counts);
 
-    hipLaunchKernelGGL(k_norm, dim3(1), dim3(1024), 0, st, n, d, e, scal, info, info_user);
+    {
+        ProfScope ps(kNorm, st);
+        hipLaunchKernelGGL(k_norm, dim3(1), dim3(1024), 0, st, n, d, e, scal, info, info_user);
+    }
     BRDC_LAUNCH_CHECK();
     if (n <= 1) return BRDC_STATUS_SUCCESS;
-    hipLaunchKernelGGL(k_prep_e, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, d, e, scal, ew);
+    {
+        ProfScope ps(kPrepE, st);
+        hipLaunchKernelGGL(k_prep_e, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, d, e, scal,
+                           ew);
+    }
     BRDC_LAUNCH_CHECK();
 
     const int levels = L.levels;
@@ -135,8 +180,11 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     double* slo[2] = {at<double>(ws, L.slo[0]), at<double>(ws, L.slo[1])};
     double* shi[2] = {at<double>(ws, L.shi[0]), at<double>(ws, L.shi[1])};
     const int single = (levels == 0);
-    hipLaunchKernelGGL(k_leaf, dim3(1 << levels), dim3(kWave), 0, st, n, levels, d, ew, scal,
-                       single ? d : sd[cur], slo[cur], shi[cur], info, single, info_user);
+    {
+        ProfScope ps(kLeaf, st);
+        hipLaunchKernelGGL(k_leaf, dim3(1 << levels), dim3(kWave), 0, st, n, levels, d, ew, scal,
+                           single ? d : sd[cur], slo[cur], shi[cur], info, single, info_user);
+    }
     BRDC_LAUNCH_CHECK();
     if (single) return BRDC_STATUS_SUCCESS;
 
@@ -175,37 +223,44 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         const int nxt = cur ^ 1;
         const int root = (depth == 0);
 
-        hipLaunchKernelGGL(k_merge_prepare, grid, dim3(kThreads), 0, st, n, depth, bpm, sd[cur],
-                           slo[cur], shi[cur], ew, md, mz, mq1, mq2);
-        BRDC_LAUNCH_CHECK();
-        if (maxsz <= 512) {
-            hipLaunchKernelGGL(k_deflate<64>, dim3(merges), dim3(64), 0, st, n, depth, ew, md, mz,
-                               mq1, mq2, P, rend, fl, ad, az, aq1, aq2, gd, gq1, gq2, td, tq1,
-                               tq2, counts);
-        } else if (maxsz <= 8192) {
-            hipLaunchKernelGGL(k_deflate<256>, dim3(merges), dim3(256), 0, st, n, depth, ew, md,
-                               mz, mq1, mq2, P, rend, fl, ad, az, aq1, aq2, gd, gq1, gq2, td, tq1,
-                               tq2, counts);
-        } else {
-            hipLaunchKernelGGL(k_deflate<1024>, dim3(merges), dim3(1024), 0, st, n, depth, ew, md,
-                               mz, mq1, mq2, P, rend, fl, ad, az, aq1, aq2, gd, gq1, gq2, td, tq1,
-                               tq2, counts);
+        {
+            ProfScope ps(kPrepare, st);
+            hipLaunchKernelGGL(k_merge_prepare, grid, dim3(kThreads), 0, st, n, depth, bpm,
+                               sd[cur], slo[cur], shi[cur], ew, md, mz, mq1, mq2);
         }
         BRDC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_secular, grid, dim3(kThreads), 0, st, n, depth, bpm, ew, counts, ad,
-                           az, rlam, rtau, rn1, rn2, rorg, counts, info);
+        {
+            ProfScope ps(kDeflate, st);
+            launch_deflate(maxsz, merges, st, n, depth, ew, md, mz, mq1, mq2, P, rend, fl, ad, az,
+                           aq1, aq2, gd, gq1, gq2, td, tq1, tq2, counts);
+        }
+        BRDC_LAUNCH_CHECK();
+        {
+            ProfScope ps(kSecular, st);
+            hipLaunchKernelGGL(k_secular, grid, dim3(kThreads), 0, st, n, depth, bpm, ew, counts,
+                               ad, az, rlam, rtau, rn1, rn2, rorg, counts, info);
+        }
         BRDC_LAUNCH_CHECK();
         if (!root) {
-            hipLaunchKernelGGL(k_wprod, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad, az,
-                               rtau, rn1, rn2, rorg, w);
+            {
+                ProfScope ps(kWprod, st);
+                hipLaunchKernelGGL(k_wprod, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad,
+                                   az, rtau, rn1, rn2, rorg, w);
+            }
             BRDC_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_rows, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad, aq1,
-                               aq2, w, rtau, rn1, rn2, rorg, qo1, qo2);
+            {
+                ProfScope ps(kRows, st);
+                hipLaunchKernelGGL(k_rows, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad,
+                                   aq1, aq2, w, rtau, rn1, rn2, rorg, qo1, qo2);
+            }
             BRDC_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_finalize, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, rlam,
-                           qo1, qo2, gd, gq1, gq2, td, tq1, tq2, root ? d : sd[nxt], slo[nxt],
-                           shi[nxt], root, scal, info, info_user);
+        {
+            ProfScope ps(kFinalize, st);
+            hipLaunchKernelGGL(k_finalize, grid, dim3(kThreads), 0, st, n, depth, bpm, counts,
+                               rlam, qo1, qo2, gd, gq1, gq2, td, tq1, tq2, root ? d : sd[nxt],
+                               slo[nxt], shi[nxt], root, scal, info, info_user);
+        }
         BRDC_LAUNCH_CHECK();
         cur = nxt;
     }
@@ -296,6 +351,39 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
         }
     }
     return total;
+}
+
+int brdc_profile_begin(void) {
+    for (auto& m : g_prof.marks) {
+        (void)hipEventDestroy(m.second.first);
+        (void)hipEventDestroy(m.second.second);
+    }
+    g_prof.marks.clear();
+    g_prof.on = true;
+    return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_profile_end(double* ms_out, int* launches_out, int nclasses) {
+    g_prof.on = false;
+    int status = BRDC_STATUS_SUCCESS;
+    std::vector<double> ms(kNumClasses, 0.0);
+    std::vector<int> cnt(kNumClasses, 0);
+    for (auto& m : g_prof.marks) {
+        float t = 0.0f;
+        if (hipEventSynchronize(m.second.second) != hipSuccess ||
+            hipEventElapsedTime(&t, m.second.first, m.second.second) != hipSuccess)
+            status = BRDC_STATUS_HIP_ERROR;
+        ms[m.first] += t;
+        cnt[m.first] += 1;
+        (void)hipEventDestroy(m.second.first);
+        (void)hipEventDestroy(m.second.second);
+    }
+    g_prof.marks.clear();
+    for (int c = 0; c < nclasses && c < kNumClasses; c++) {
+        if (ms_out) ms_out[c] = ms[c];
+        if (launches_out) launches_out[c] = cnt[c];
+    }
+    return status;
 }
 
 int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork, int liwork) {
