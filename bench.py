@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--n", type=int, default=49152)
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch kernels one by one instead of replaying a captured graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -112,26 +114,41 @@ def main():
     w_dev = {g: torch.empty_like(d_dev[g]) for g in GENS}
     solver = brdc.Solver(n, device=dev, leaf_max=args.leaf)
     stream = torch.cuda.current_stream()
+    # one solver (workspace + info word) and one captured graph per matrix:
+    # the graph replays the copy of D into the output buffer and every kernel
+    graphs = {}
+    if not args.eager:
+        for g in GENS:
+            graphs[g] = brdc.GraphSolve(brdc.Solver(n, device=dev, leaf_max=args.leaf),
+                                        d_dev[g], e_dev[g])
 
     def step():
-        for g in GENS:
-            w_dev[g].copy_(d_dev[g])
-            solver.solve(w_dev[g], e_dev[g], stream)
+        if graphs:
+            for g in GENS:
+                graphs[g].run()
+        else:
+            for g in GENS:
+                w_dev[g].copy_(d_dev[g])
+                solver.solve(w_dev[g], e_dev[g], stream)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    # correctness of the results the timed region will reproduce
+    # correctness of the results the timed region reproduces
     step()
     torch.cuda.synchronize()
     infos = {}
     outs = {}
     for g in GENS:
-        solver.solve(w_dev[g].copy_(d_dev[g]), e_dev[g], stream)
-        torch.cuda.synchronize()
-        infos[g] = int(solver.info.item())
-        outs[g] = w_dev[g].cpu().numpy()
+        if graphs:
+            infos[g] = int(graphs[g].solver.info.item())
+            outs[g] = graphs[g].w.cpu().numpy()
+        else:
+            solver.solve(w_dev[g].copy_(d_dev[g]), e_dev[g], stream)
+            torch.cuda.synchronize()
+            infos[g] = int(solver.info.item())
+            outs[g] = w_dev[g].cpu().numpy()
 
     # timed region: exactly `steps` steps between two synchronisations
     torch.cuda.synchronize()
@@ -148,18 +165,31 @@ def main():
     ms_per_step = gpu_ms / args.steps
     value = ms_per_step / len(GENS)
 
-    # per-generator ms and per-kernel breakdown (separate pass, same work)
-    per_gen = {}
-    for g in GENS:
+    # per-generator ms (graph replay and eager launches) and per-kernel
+    # breakdown (separate passes over the same work)
+    def timed(fn, reps):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        for _ in range(max(1, args.steps // 2)):
-            w_dev[g].copy_(d_dev[g])
-            solver.solve(w_dev[g], e_dev[g], stream)
+        for _ in range(reps):
+            fn()
         b.record(stream)
         torch.cuda.synchronize()
-        per_gen[g] = a.elapsed_time(b) / max(1, args.steps // 2)
+        return a.elapsed_time(b) / reps
+
+    reps = max(1, args.steps // 2)
+    per_gen = {}
+    per_gen_eager = {}
+    for g in GENS:
+        if graphs:
+            per_gen[g] = timed(graphs[g].run, reps)
+
+        def eager(g=g):
+            w_dev[g].copy_(d_dev[g])
+            solver.solve(w_dev[g], e_dev[g], stream)
+        per_gen_eager[g] = timed(eager, reps)
+    if not graphs:
+        per_gen = per_gen_eager
 
     prof = brdc.profile_kernels(solver, [(w_dev[g], d_dev[g], e_dev[g]) for g in GENS], stream)
 
@@ -223,7 +253,10 @@ def main():
         "config": {"workload": f"plain BR values-only, N={n}, one solve per generator "
                                f"({', '.join(GENS)}) per step",
                    "n": n, "generators": list(GENS), "leaf_max": solver.opts.leaf_max or 32,
+                   "launch": "eager" if args.eager else "hipGraph replay",
                    "per_generator_ms": {g: round(v, 4) for g, v in per_gen.items()},
+                   "per_generator_ms_eager_launch": {g: round(v, 4)
+                                                     for g, v in per_gen_eager.items()},
                    "workspace_bytes": solver.bytes},
         "accuracy": {"pass": ok, "worst_rel_err": worst, "per_generator": acc},
         "roofline": roof,

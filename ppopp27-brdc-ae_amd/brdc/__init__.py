@@ -23,7 +23,7 @@ from . import _lib
 from ._lib import BrdcError, options
 
 __all__ = ["dstebr", "dstebr_device", "workspace_bytes", "merge_stats", "Solver", "BrdcError",
-           "profile_kernels",
+           "profile_kernels", "GraphSolve",
            "options", "query"]
 
 
@@ -183,3 +183,35 @@ def profile_kernels(solver: Solver, problems, stream=None):
             kern[name] = {"ms": float(tot_ms[i]), "launches": int(tot_n[i]),
                           "avg_us": float(tot_ms[i] * 1e3 / tot_n[i])}
     return {"kernels": kern, "flops": flops, "bytes": nbytes}
+
+
+class GraphSolve:
+    """One solve captured into a HIP graph (via torch's graph capture).
+
+    The device entry point does no host synchronisation and no allocation, so
+    the whole launch sequence (copy of d into the solver's output buffer, then
+    every level's kernels) records into one graph; ``run()`` replays it on the
+    current stream.  Inputs are read from the tensors given at capture time.
+    """
+
+    def __init__(self, solver: Solver, d, e):
+        import torch
+        self.solver = solver
+        self.d, self.e = d, e
+        self.w = torch.empty_like(d)
+        self.graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=d.device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm-up outside capture
+            self.w.copy_(d)
+            solver.solve(self.w, e)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize(d.device)
+        with torch.cuda.graph(self.graph):
+            self.w.copy_(d)
+            solver.solve(self.w, e)
+        torch.cuda.synchronize(d.device)
+
+    def run(self):
+        self.graph.replay()
+        return self.w

@@ -22,6 +22,7 @@ namespace brdc {
 namespace {
 
 constexpr int kDefaultLeaf = 32;
+constexpr int kWideGrid = 8;
 constexpr size_t kAlign = 256;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
@@ -31,10 +32,10 @@ struct Layout {
     size_t scal, ew;
     size_t sd[2], slo[2], shi[2];
     size_t md, mz, mq1, mq2;
-    size_t ad, az, aq1, aq2;
+    size_t ad, az, adz, aq;
     size_t gd, gq1, gq2;
     size_t td, tq1, tq2;
-    size_t rlam, rtau, rn1, rn2;
+    size_t rlam, rtau, rn1, rn2, rdorg;
     size_t w, qo1, qo2;
     size_t rorg, P, rend, fl;
     size_t counts, info;
@@ -68,8 +69,8 @@ Layout make_layout(int n, int leaf) {
     L.mq2 = take(nd);
     L.ad = take(nd);
     L.az = take(nd);
-    L.aq1 = take(nd);
-    L.aq2 = take(nd);
+    L.adz = take(2 * nd);
+    L.aq = take(2 * nd);
     L.gd = take(nd);
     L.gq1 = take(nd);
     L.gq2 = take(nd);
@@ -80,6 +81,7 @@ Layout make_layout(int n, int leaf) {
     L.rtau = take(nd);
     L.rn1 = take(nd);
     L.rn2 = take(nd);
+    L.rdorg = take(nd);
     L.w = take(nd);
     L.qo1 = take(nd);
     L.qo2 = take(nd);
@@ -182,7 +184,9 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     const int single = (levels == 0);
     {
         ProfScope ps(kLeaf, st);
-        hipLaunchKernelGGL(k_leaf, dim3(1 << levels), dim3(kWave), 0, st, n, levels, d, ew, scal,
+        const int leaf_rows = max_node_size(n, levels);
+        auto kern = (leaf_rows <= 16) ? k_leaf<16> : (leaf_rows <= 32) ? k_leaf<32> : k_leaf<64>;
+        hipLaunchKernelGGL(kern, dim3(1 << levels), dim3(kWave), 0, st, n, levels, d, ew, scal,
                            single ? d : sd[cur], slo[cur], shi[cur], info, single, info_user);
     }
     BRDC_LAUNCH_CHECK();
@@ -194,8 +198,8 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     double* mq2 = at<double>(ws, L.mq2);
     double* ad = at<double>(ws, L.ad);
     double* az = at<double>(ws, L.az);
-    double* aq1 = at<double>(ws, L.aq1);
-    double* aq2 = at<double>(ws, L.aq2);
+    double2* adz = at<double2>(ws, L.adz);
+    double2* aq = at<double2>(ws, L.aq);
     double* gd = at<double>(ws, L.gd);
     double* gq1 = at<double>(ws, L.gq1);
     double* gq2 = at<double>(ws, L.gq2);
@@ -206,6 +210,7 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     double* rtau = at<double>(ws, L.rtau);
     double* rn1 = at<double>(ws, L.rn1);
     double* rn2 = at<double>(ws, L.rn2);
+    double* rdorg = at<double>(ws, L.rdorg);
     double* w = at<double>(ws, L.w);
     double* qo1 = at<double>(ws, L.qo1);
     double* qo2 = at<double>(ws, L.qo2);
@@ -219,6 +224,10 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         const int maxsz = max_node_size(n, depth);
         const int bpm = (maxsz + kThreads - 1) / kThreads;
         const dim3 grid(merges * bpm);
+        // the O(K^2) kernels get kWideGrid threads per row so that several
+        // lanes can share one root or pole (lanes_per_item)
+        const int bpmw = bpm * kWideGrid;
+        const dim3 gridw(merges * bpmw);
         int* counts = counts_all + 4 * (merges - 1);
         const int nxt = cur ^ 1;
         const int root = (depth == 0);
@@ -232,26 +241,26 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         {
             ProfScope ps(kDeflate, st);
             launch_deflate(maxsz, merges, st, n, depth, ew, md, mz, mq1, mq2, P, rend, fl, ad, az,
-                           aq1, aq2, gd, gq1, gq2, td, tq1, tq2, counts);
+                           adz, aq, gd, gq1, gq2, td, tq1, tq2, counts);
         }
         BRDC_LAUNCH_CHECK();
         {
             ProfScope ps(kSecular, st);
-            hipLaunchKernelGGL(k_secular, grid, dim3(kThreads), 0, st, n, depth, bpm, ew, counts,
-                               ad, az, rlam, rtau, rn1, rn2, rorg, counts, info);
+            hipLaunchKernelGGL(k_secular, gridw, dim3(kThreads), 0, st, n, depth, bpmw, ew, counts,
+                               ad, az, adz, rlam, rtau, rn1, rn2, rdorg, rorg, counts, info);
         }
         BRDC_LAUNCH_CHECK();
         if (!root) {
             {
                 ProfScope ps(kWprod, st);
-                hipLaunchKernelGGL(k_wprod, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad,
-                                   az, rtau, rn1, rn2, rorg, w);
+                hipLaunchKernelGGL(k_wprod, gridw, dim3(kThreads), 0, st, n, depth, bpmw, counts,
+                                   ad, az, rtau, rn1, rn2, rdorg, w);
             }
             BRDC_LAUNCH_CHECK();
             {
                 ProfScope ps(kRows, st);
-                hipLaunchKernelGGL(k_rows, grid, dim3(kThreads), 0, st, n, depth, bpm, counts, ad,
-                                   aq1, aq2, w, rtau, rn1, rn2, rorg, qo1, qo2);
+                hipLaunchKernelGGL(k_rows, gridw, dim3(kThreads), 0, st, n, depth, bpmw, counts,
+                                   ad, aq, w, rtau, rn1, rn2, rdorg, qo1, qo2);
             }
             BRDC_LAUNCH_CHECK();
         }

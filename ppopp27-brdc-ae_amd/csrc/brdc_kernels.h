@@ -25,7 +25,6 @@
 namespace brdc {
 
 constexpr int kWave = 64;
-constexpr int kLds = 65;          // padded LDS row stride of the leaf matrix
 constexpr int kMaxLeaf = 64;
 constexpr int kJacobiSweeps = 40;
 constexpr int kSpecCap = 256;     // speculative Givens-run walk bound
@@ -159,45 +158,51 @@ __global__ void k_prep_e(int n, const double* __restrict__ d, const double* __re
 }
 
 // ---------------------------------------------------------------------------
-// Leaf eigensolve: one wavefront per leaf (<= 64 rows) holding the dense leaf
-// in LDS.  Cyclic Jacobi in round-robin (tournament) order: each round
-// annihilates m/2 disjoint (p, q) pairs at once, rows updated one column per
-// lane, then columns one row per lane.  Only the first and last rows of the
-// accumulated rotation product are kept (the role of DSTEQR_BOUNDARY_BR,
+// Leaf eigensolve: one wavefront per leaf (<= MAXM rows) holding the dense
+// leaf in LDS.  Cyclic Jacobi in round-robin (tournament) order: each round
+// picks m/2 disjoint (p, q) pairs, computes their rotations, and applies
+// A <- J^T A J as independent 2x2 block updates -- block (a, b) of the pair
+// structure becomes R_a^T A[P_a, P_b] R_b -- so a round is one LDS pass with
+// no intermediate barrier.  Only the first and last rows of the accumulated
+// rotation product are kept (the role of DSTEQR_BOUNDARY_BR,
 // dlaed0_br.f:354-371).  The rank-one cut corrections (dlaed0_br.f:95-103)
 // are applied to the leaf's end rows on load.
+template <int MAXM>
 __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* __restrict__ d_in,
                                                 const double* __restrict__ ew,
                                                 const double* __restrict__ scal, double* out_d,
                                                 double* out_lo, double* out_hi, int* info,
                                                 int single, int* info_user) {
-    __shared__ double A[kMaxLeaf * kLds];
-    __shared__ double V0[kMaxLeaf], V1[kMaxLeaf];
-    __shared__ double RC[kMaxLeaf / 2], RS[kMaxLeaf / 2];
-    __shared__ int RP[kMaxLeaf / 2], RQ[kMaxLeaf / 2];
+    constexpr int LD = MAXM + 1;
+    __shared__ double A[MAXM * LD];
+    __shared__ double V0[MAXM], V1[MAXM];
+    __shared__ double RC[MAXM / 2], RS[MAXM / 2];
+    __shared__ int RP[MAXM / 2], RQ[MAXM / 2];
     __shared__ int changed;
     int s, m;
     node_geom(n, depth, blockIdx.x, &s, &m);
     const int lane = threadIdx.x;
     const double scale = scal[0];
-    for (int idx = lane; idx < kMaxLeaf * kLds; idx += kWave) A[idx] = 0.0;
+    for (int idx = lane; idx < MAXM * LD; idx += kWave) A[idx] = 0.0;
     __syncthreads();
     if (lane < m) {
         double di = d_in[s + lane] * scale;
         if (lane == 0 && s > 0) di -= fabs(ew[s - 1]);
         if (lane == m - 1 && s + m < n) di -= fabs(ew[s + m - 1]);
-        A[lane * kLds + lane] = di;
+        A[lane * LD + lane] = di;
         if (lane < m - 1) {
             const double el = ew[s + lane];
-            A[lane * kLds + lane + 1] = el;
-            A[(lane + 1) * kLds + lane] = el;
+            A[lane * LD + lane + 1] = el;
+            A[(lane + 1) * LD + lane] = el;
         }
         V0[lane] = (lane == 0) ? 1.0 : 0.0;
         V1[lane] = (lane == m - 1) ? 1.0 : 0.0;
     }
     __syncthreads();
+    // an odd m gets a zero dummy row/column m; its pairs never rotate
     const int mm = m + (m & 1);
     const int npair = mm / 2;
+    const int nblk = npair * npair;
     bool converged = (m <= 1);
     for (int sweep = 0; sweep < kJacobiSweeps && !converged; sweep++) {
         if (lane == 0) changed = 0;
@@ -210,8 +215,8 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
                 const int p = min(a, b), q = max(a, b);
                 double c = 1.0, sn = 0.0;
                 if (q < m) {
-                    const double apq = A[p * kLds + q];
-                    const double app = A[p * kLds + p], aqq = A[q * kLds + q];
+                    const double apq = A[p * LD + q];
+                    const double app = A[p * LD + p], aqq = A[q * LD + q];
                     if (apq != 0.0 && fabs(apq) > kEps * sqrt(fabs(app)) * sqrt(fabs(aqq))) {
                         const double theta = (aqq - app) / (2.0 * apq);
                         double t;
@@ -230,28 +235,27 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
                 RQ[lane] = q;
             }
             __syncthreads();
-            if (lane < m) {  // A <- J^T A: rows p, q of column `lane`
-                for (int k = 0; k < npair; k++) {
-                    const double sn = RS[k];
-                    if (sn == 0.0) continue;
-                    const double c = RC[k];
-                    const int p = RP[k], q = RQ[k];
-                    const double x = A[p * kLds + lane], y = A[q * kLds + lane];
-                    A[p * kLds + lane] = c * x - sn * y;
-                    A[q * kLds + lane] = sn * x + c * y;
+            for (int blk = lane; blk < nblk; blk += kWave) {
+                const int a = blk / npair, b = blk - a * npair;
+                const double sa = RS[a], sb = RS[b];
+                if (sa == 0.0 && sb == 0.0) continue;
+                const double ca = RC[a], cb = RC[b];
+                const int pa = RP[a], qa = RQ[a], pb = RP[b], qb = RQ[b];
+                const double x00 = A[pa * LD + pb], x01 = A[pa * LD + qb];
+                const double x10 = A[qa * LD + pb], x11 = A[qa * LD + qb];
+                // rows (J_a^T), then columns (J_b)
+                const double y00 = ca * x00 - sa * x10, y01 = ca * x01 - sa * x11;
+                const double y10 = sa * x00 + ca * x10, y11 = sa * x01 + ca * x11;
+                double z00 = cb * y00 - sb * y01, z01 = sb * y00 + cb * y01;
+                double z10 = cb * y10 - sb * y11, z11 = sb * y10 + cb * y11;
+                if (a == b) {  // the annihilated pair
+                    z01 = 0.0;
+                    z10 = 0.0;
                 }
-            }
-            __syncthreads();
-            if (lane < m) {  // A <- A J: columns p, q of row `lane`
-                for (int k = 0; k < npair; k++) {
-                    const double sn = RS[k];
-                    if (sn == 0.0) continue;
-                    const double c = RC[k];
-                    const int p = RP[k], q = RQ[k];
-                    const double x = A[lane * kLds + p], y = A[lane * kLds + q];
-                    A[lane * kLds + p] = c * x - sn * y;
-                    A[lane * kLds + q] = sn * x + c * y;
-                }
+                A[pa * LD + pb] = z00;
+                A[pa * LD + qb] = z01;
+                A[qa * LD + pb] = z10;
+                A[qa * LD + qb] = z11;
             }
             if (lane < npair && RS[lane] != 0.0) {  // first/last rows of V <- V J
                 const double c = RC[lane], sn = RS[lane];
@@ -265,19 +269,14 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
                 V1[q] = sn * x + c * y;
             }
             __syncthreads();
-            if (lane < npair && RS[lane] != 0.0) {
-                A[RP[lane] * kLds + RQ[lane]] = 0.0;
-                A[RQ[lane] * kLds + RP[lane]] = 0.0;
-            }
-            __syncthreads();
         }
         converged = (changed == 0);
         __syncthreads();
     }
     if (!converged && lane == 0) report_failure(info, n, s, m);
     // sort ascending, carrying the two rows
-    __shared__ double L[kMaxLeaf];
-    if (lane < m) L[lane] = A[lane * kLds + lane];
+    __shared__ double L[MAXM];
+    if (lane < m) L[lane] = A[lane * LD + lane];
     __syncthreads();
     if (lane < m) {
         const double x = L[lane];
@@ -407,7 +406,7 @@ __global__ __launch_bounds__(T) void k_deflate(
     int n, int depth, const double* __restrict__ ew, double* __restrict__ md,
     double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2, int* __restrict__ P,
     int* __restrict__ rend, int* __restrict__ fl, double* __restrict__ ad, double* __restrict__ az,
-    double* __restrict__ aq1, double* __restrict__ aq2, double* __restrict__ gd,
+    double2* __restrict__ adz, double2* __restrict__ aq, double* __restrict__ gd,
     double* __restrict__ gq1, double* __restrict__ gq2, double* __restrict__ td,
     double* __restrict__ tq1, double* __restrict__ tq2, int* __restrict__ counts) {
     __shared__ int lds[T / kWave + 2];
@@ -424,6 +423,7 @@ __global__ __launch_bounds__(T) void k_deflate(
     const int seg = (sz + T - 1) / T;
     const int p0 = min(sz, tid * seg), p1 = min(sz, p0 + seg);
     int cnt = 0;
+#pragma unroll 8
     for (int p = p0; p < p1; p++) {
         const bool keep = rho * fabs(mz[s + p]) > tol;
         cnt += keep;
@@ -578,6 +578,7 @@ __global__ __launch_bounds__(T) void k_deflate(
 
     // ---- D: compaction of kept poles and tiny-deflated entries
     int na = 0, nt = 0;
+#pragma unroll 8
     for (int p = p0; p < p1; p++) {
         const int f = fl[s + p];
         na += (f == 0);
@@ -589,10 +590,11 @@ __global__ __launch_bounds__(T) void k_deflate(
     for (int p = p0; p < p1; p++) {
         const int f = fl[s + p];
         if (f == 0) {
-            ad[s + aoff] = md[s + p];
-            az[s + aoff] = mz[s + p];
-            aq1[s + aoff] = mq1[s + p];
-            aq2[s + aoff] = mq2[s + p];
+            const double dv = md[s + p], zv = mz[s + p];
+            ad[s + aoff] = dv;
+            az[s + aoff] = zv;
+            adz[s + aoff] = make_double2(dv, zv * zv);
+            aq[s + aoff] = make_double2(mq1[s + p], mq2[s + p]);
             aoff++;
         } else if (f == 2) {
             td[s + toff] = md[s + p];
@@ -610,28 +612,76 @@ __global__ __launch_bounds__(T) void k_deflate(
 }
 
 // ---------------------------------------------------------------------------
-// Merge step 3: secular roots (DLAED9_BR's DLAED4_BR loop, dlaed9_br.f:181-262),
-// one lane per root; per root the compact state (org, tau, near1, near2).
-struct GlobalPoles {
-    const double* __restrict__ dp;
-    const double* __restrict__ zp;
-    __device__ __forceinline__ double d(int j) const { return dp[j]; }
-    __device__ __forceinline__ double z(int j) const { return zp[j]; }
+// Merge steps 3-5 share one work layout: L lanes per item (root or pole),
+// L a power of two chosen from the merge's K at run time so the level's grid
+// is filled and each lane streams about K/L poles or roots.  Lane c of an
+// item takes the strided indices c, c+L, ... (coalesced across the L lanes);
+// partial sums meet in an xor butterfly inside the item's lane group.
+__device__ __forceinline__ int lanes_per_item(int K, int capacity) {
+    int L = kWave;
+    while (L > 1 && (K * L > capacity || K < 16 * L)) L >>= 1;
+    return L;
+}
+
+template <class T>
+__device__ __forceinline__ T group_sum(T v, int L) {
+    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Merge step 3: secular roots (DLAED9_BR's DLAED4_BR loop, dlaed9_br.f:181-262);
+// per root the compact state (org, tau, near1, near2) plus d[org].
+struct LaneEval {
+    const double2* __restrict__ dz;  // (d_j, z_j^2)
+    int K, L, c;
+    __device__ __forceinline__ double d(int j) const { return dz[j].x; }
+    __device__ __forceinline__ double zsq(int j) const { return dz[j].y; }
+    __device__ SecSums sums(double dorg, double tau, int ex_lo, int nex) const {
+        double S = 0.0, A = 0.0, DP = 0.0, DF = 0.0, J = 0.0;
+        double jd = (double)c;
+        const double dL = (double)L;
+#pragma unroll 2
+        for (int j = c; j < K; j += L) {
+            const double2 p = dz[j];
+            const double delta = (p.x - dorg) - tau;
+            const double r = rcp_nr(delta);
+            double term = p.y * r;
+            if ((unsigned)(j - ex_lo) < (unsigned)nex) term = 0.0;
+            S += term;
+            A += fabs(term);
+            const double x = term * fabs(r);
+            DP += fmin(x, 0.0);
+            DF += fmax(x, 0.0);
+            J = fma(jd, term, J);
+            jd += dL;
+        }
+        SecSums out;
+        out.S = group_sum(S, L);
+        out.A = group_sum(A, L);
+        out.dpsi = -group_sum(DP, L);
+        out.dphi = group_sum(DF, L);
+        out.J = group_sum(J, L);
+        return out;
+    }
 };
 
 __global__ __launch_bounds__(kThreads) void k_secular(
     int n, int depth, int bpm, const double* __restrict__ ew, const int* counts,
-    const double* __restrict__ ad, const double* __restrict__ az, double* __restrict__ rlam,
-    double* __restrict__ rtau, double* __restrict__ rn1, double* __restrict__ rn2,
+    const double* __restrict__ ad, const double* __restrict__ az,
+    const double2* __restrict__ adz, double* __restrict__ rlam, double* __restrict__ rtau,
+    double* __restrict__ rn1, double* __restrict__ rn2, double* __restrict__ rdorg,
     int* __restrict__ rorg, int* counts_rw, int* info) {
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
     const int K = counts[4 * merge];
-    if (part * kThreads >= K) return;
+    const int L = lanes_per_item(K, bpm * kThreads);
+    const int per_block = kThreads / L;
+    if (part * per_block >= K) return;
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
     const int cut = sz / 2;
     const double rho = fabs(2.0 * ew[s + cut - 1]);
-    const int j = part * kThreads + threadIdx.x;
+    const int c = threadIdx.x & (L - 1);
+    const int j = part * per_block + threadIdx.x / L;
     int iters = 0;
     if (j < K) {
         if (K == 1) {
@@ -641,6 +691,7 @@ __global__ __launch_bounds__(kThreads) void k_secular(
             rn1[s] = 1.0;  // secular eigenvector (1)
             rn2[s] = 0.0;
             rorg[s] = 0;
+            rdorg[s] = ad[s];
             iters = 1;
         } else if (K == 2) {
             double lam, v0, v1, tau;
@@ -650,109 +701,117 @@ __global__ __launch_bounds__(kThreads) void k_secular(
             rn1[s + j] = v0;  // secular eigenvector (v0, v1)
             rn2[s + j] = v1;
             rorg[s + j] = 0;
+            rdorg[s + j] = ad[s];
             iters = 1;
         } else {
-            GlobalPoles P{ad + s, az + s};
-            const RootState r = secular_root(P, K, j, rho);
-            rlam[s + j] = r.lambda;
-            rtau[s + j] = r.tau;
-            rn1[s + j] = r.near1;
-            rn2[s + j] = r.near2;
-            rorg[s + j] = r.org;
-            iters = r.iters;
-            if (r.info) report_failure(info, n, s, sz);
+            const LaneEval E{adz + s, K, L, c};
+            const RootState r = secular_root(E, K, j, rho);
+            if (c == 0) {
+                rlam[s + j] = r.lambda;
+                rtau[s + j] = r.tau;
+                rn1[s + j] = r.near1;
+                rn2[s + j] = r.near2;
+                rorg[s + j] = r.org;
+                rdorg[s + j] = ad[s + r.org];
+                iters = r.iters;
+                if (r.info) report_failure(info, n, s, sz);
+            }
         }
     }
-    // one atomic per wave
     for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
     if ((threadIdx.x & 63) == 0 && iters) atomicAdd(&counts_rw[4 * merge + 3], iters);
 }
 
-// delta_i of root j from the compact state: the pole-adjacent entries come
-// from near1/near2, every other one is (d_i - d_org) - tau (dlaed9_br.f:272-331)
-__device__ __forceinline__ double compact_delta(int i, int j, int K, double di, double dorg,
-                                                double tau, double nr1, double nr2) {
-    const int a = (j == K - 1) ? K - 2 : j;
-    if (i == a) return nr1;
-    if (i == a + 1) return nr2;
-    return (di - dorg) - tau;
-}
-
-// Merge step 4: product-form update vector (dlaed9_br.f:218-266): one lane per
-// pole i, streaming the roots: w_i = sign(z_i) sqrt(-prod_j delta_j(i) /
-// prod_{j != i} (d_i - d_j)).
-__global__ __launch_bounds__(kThreads) void k_wprod(int n, int depth, int bpm,
-                                                    const int* __restrict__ counts,
-                                                    const double* __restrict__ ad,
-                                                    const double* __restrict__ az,
-                                                    const double* __restrict__ rtau,
-                                                    const double* __restrict__ rn1,
-                                                    const double* __restrict__ rn2,
-                                                    const int* __restrict__ rorg,
-                                                    double* __restrict__ w) {
+// Merge step 4: product-form update vector (dlaed9_br.f:218-266): L lanes per
+// pole i streaming the roots j: w_i = sign(z_i) sqrt(-prod_j delta_j(i) /
+// prod_{j != i} (d_i - d_j)).  delta_j(i) comes from root j's compact state;
+// the two pole-adjacent entries are root j's near1/near2 (dlaed9_br.f:272-331).
+__global__ __launch_bounds__(kThreads) void k_wprod(
+    int n, int depth, int bpm, const int* __restrict__ counts, const double* __restrict__ ad,
+    const double* __restrict__ az, const double* __restrict__ rtau,
+    const double* __restrict__ rn1, const double* __restrict__ rn2,
+    const double* __restrict__ rdorg, double* __restrict__ w) {
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
     const int K = counts[4 * merge];
-    if (K <= 2 || part * kThreads >= K) return;
+    if (K <= 2) return;
+    const int L = lanes_per_item(K, bpm * kThreads);
+    const int per_block = kThreads / L;
+    if (part * per_block >= K) return;
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
-    const int i = part * kThreads + threadIdx.x;
-    if (i >= K) return;
-    const double di = ad[s + i];
+    const int c = threadIdx.x & (L - 1);
+    const int i = part * per_block + threadIdx.x / L;
+    if (i >= K) return;  // whole lane groups leave together
+    const double* __restrict__ d = ad + s;
+    const double di = d[i];
     double prod = 1.0;
-    for (int j = 0; j < K; j++) {
-        const int org = rorg[s + j];
-        const double dl = compact_delta(i, j, K, di, ad[s + org], rtau[s + j], rn1[s + j], rn2[s + j]);
-        if (j == i) prod *= dl;
-        else prod *= dl / (di - ad[s + j]);
+    for (int j = c; j < K; j += L) {
+        double dl = (di - rdorg[s + j]) - rtau[s + j];
+        const int a = (j == K - 1) ? K - 2 : j;
+        if ((unsigned)(i - a) <= 1u) dl = (i == a) ? rn1[s + j] : rn2[s + j];
+        prod *= (j == i) ? dl : dl * rcp_nr(di - d[j]);
     }
-    w[s + i] = copysign(sqrt(-prod), az[s + i]);
+    for (int o = 1; o < L; o <<= 1) prod *= __shfl_xor(prod, o, kWave);
+    if (c == 0) w[s + i] = copysign(sqrt(-prod), az[s + i]);
 }
 
 // Merge step 5: the two boundary rows of the merged eigenvector block
-// (dlaed9_br.f:272-331): one lane per root j, streaming the poles.
-__global__ __launch_bounds__(kThreads) void k_rows(int n, int depth, int bpm,
-                                                   const int* __restrict__ counts,
-                                                   const double* __restrict__ ad,
-                                                   const double* __restrict__ aq1,
-                                                   const double* __restrict__ aq2,
-                                                   const double* __restrict__ w,
-                                                   const double* __restrict__ rtau,
-                                                   const double* __restrict__ rn1,
-                                                   const double* __restrict__ rn2,
-                                                   const int* __restrict__ rorg,
-                                                   double* __restrict__ qo1,
-                                                   double* __restrict__ qo2) {
+// (dlaed9_br.f:272-331): L lanes per root j streaming the poles i,
+// x_i = w_i / delta_j(i), row_r(j) = sum_i q_r(i) x_i / ||x||.
+__global__ __launch_bounds__(kThreads) void k_rows(
+    int n, int depth, int bpm, const int* __restrict__ counts, const double* __restrict__ ad,
+    const double2* __restrict__ aq, const double* __restrict__ w,
+    const double* __restrict__ rtau, const double* __restrict__ rn1,
+    const double* __restrict__ rn2, const double* __restrict__ rdorg,
+    double* __restrict__ qo1, double* __restrict__ qo2) {
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
     const int K = counts[4 * merge];
-    if (part * kThreads >= K) return;
+    const int L = lanes_per_item(K, bpm * kThreads);
+    const int per_block = kThreads / L;
+    if (part * per_block >= K) return;
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
-    const int j = part * kThreads + threadIdx.x;
+    const int c = threadIdx.x & (L - 1);
+    const int j = part * per_block + threadIdx.x / L;
     if (j >= K) return;
-    if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172)
+    if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172); L == 1
         const double v0 = rn1[s + j], v1 = rn2[s + j];
-        double a1 = aq1[s] * v0, a2 = aq2[s] * v0;
+        const double2 q0 = aq[s];
+        double a1 = q0.x * v0, a2 = q0.y * v0;
         if (K == 2) {
-            a1 += aq1[s + 1] * v1;
-            a2 += aq2[s + 1] * v1;
+            const double2 q1 = aq[s + 1];
+            a1 += q1.x * v1;
+            a2 += q1.y * v1;
         }
         qo1[s + j] = a1;
         qo2[s + j] = a2;
         return;
     }
     const double tau = rtau[s + j], nr1 = rn1[s + j], nr2 = rn2[s + j];
-    const double dorg = ad[s + rorg[s + j]];
+    const double dorg = rdorg[s + j];
+    const int a = (j == K - 1) ? K - 2 : j;
+    const double* __restrict__ d = ad + s;
+    const double* __restrict__ wv = w + s;
+    const double2* __restrict__ q = aq + s;
     double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
-    for (int i = 0; i < K; i++) {
-        const double dl = compact_delta(i, j, K, ad[s + i], dorg, tau, nr1, nr2);
-        const double x = w[s + i] / dl;
-        temp += x * x;
-        acc1 += aq1[s + i] * x;
-        acc2 += aq2[s + i] * x;
+#pragma unroll 2
+    for (int i = c; i < K; i += L) {
+        double dl = (d[i] - dorg) - tau;
+        if ((unsigned)(i - a) <= 1u) dl = (i == a) ? nr1 : nr2;
+        const double x = wv[i] * rcp_nr(dl);
+        const double2 qi = q[i];
+        temp = fma(x, x, temp);
+        acc1 = fma(qi.x, x, acc1);
+        acc2 = fma(qi.y, x, acc2);
     }
-    temp = sqrt(temp);
-    qo1[s + j] = acc1 / temp;
-    qo2[s + j] = acc2 / temp;
+    temp = group_sum(temp, L);
+    acc1 = group_sum(acc1, L);
+    acc2 = group_sum(acc2, L);
+    if (c == 0) {
+        const double nrm = sqrt(temp);
+        qo1[s + j] = acc1 / nrm;
+        qo2[s + j] = acc2 / nrm;
+    }
 }
 
 // Merge step 6: the node's ascending order (DLAMRG of roots with deflated

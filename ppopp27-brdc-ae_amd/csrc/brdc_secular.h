@@ -237,63 +237,58 @@ done:
     return info;
 }
 
-// Sums over all K poles with delta_j = (d_j - d_org) - tau:
-//   psi = sum_{j < lo_end} z_j^2/delta_j,  phi = sum_{j >= hi_start} z_j^2/delta_j,
-// their derivatives, and the rounding-error bound of the sequential partial
-// sums (sum of |partial sums|), which for same-signed terms equals the
-// count-weighted sum below.  One uniform loop over j for every lane of a wave.
-struct SecSums {
-    double psi, dpsi, phi, dphi, err_psi, err_phi;
-};
-
-template <class Poles>
-__host__ __device__ __forceinline__ SecSums secular_sums(const Poles& P, int K, double dorg, double tau,
-                                                int lo_end, int hi_start) {
-    SecSums s = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < K; j++) {
-        const double dj = P.d(j), zj = P.z(j);
-        const double delta = (dj - dorg) - tau;
-        const double t = zj / delta;
-        const double term = zj * t;
-        const double dterm = t * t;
-        if (j < lo_end) {
-            s.psi += term;
-            s.dpsi += dterm;
-            s.err_psi += (double)(lo_end - j) * term;
-        } else if (j >= hi_start) {
-            s.phi += term;
-            s.dphi += dterm;
-            s.err_phi += (double)(j - hi_start + 1) * term;
-        }
-    }
-    return s;
+// Reciprocal for the pole loops: hardware v_rcp_f64 refined by two Newton
+// steps (error below one ulp); the host build of this header uses 1/x.
+__host__ __device__ __forceinline__ double rcp_nr(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
 }
 
-// Root i (0-based) of the K-pole secular equation, K >= 3.
-template <class Poles>
-__host__ __device__ RootState secular_root(const Poles& P, int K, int i, double rho) {
+// One evaluation pass over the poles j outside [ex_lo, ex_lo + nex), with
+// delta_j = (d_j - d_org) - tau and term_j = z_j^2 / delta_j:
+//   S = sum term_j,  A = sum |term_j|,  J = sum j * term_j,
+//   dpsi = sum over delta_j < 0 of (z_j/delta_j)^2, dphi likewise delta_j > 0.
+// Poles below the root have delta < 0 (LAPACK's psi side) and poles above
+// delta > 0 (the phi side), so psi + phi = S and phi - psi = A; and the bound
+// LAPACK accumulates from its sequential partial sums,
+// |sum_{j<ii} psi_{<=j}| + sum_{j>ii} phi_{>=j}, equals sum_j (j - ii) term_j
+// = J - ii*S.  No per-lane index ranges: every lane runs the same loop.
+struct SecSums {
+    double S, A, dpsi, dphi, J;
+};
+
+// Root i (0-based) of the K-pole secular equation, K >= 3.  E provides
+// d(j), zsq(j) = z_j^2 and sums(d_org, tau, ex_lo, nex).
+template <class Eval>
+__host__ __device__ RootState secular_root(const Eval& E, int K, int i, double rho) {
     RootState r;
     r.info = 0;
     r.iters = 0;
     const double rhoinv = 1.0 / rho;
     if (i == K - 1) {
         // ---- last root, in (d_{K-1}, d_{K-1} + rho)
-        const int ii = K - 2;
-        const double dk = P.d(K - 1), dk1 = P.d(K - 2);
-        const double zk = P.z(K - 1), zk1 = P.z(K - 2);
+        const double dk = E.d(K - 1), dk1 = E.d(K - 2);
+        const double zk2 = E.zsq(K - 1), zk12 = E.zsq(K - 2);
         const double midpt = rho / 2.0;
         double tau, dltlb, dltub;
         {
-            SecSums s0 = secular_sums(P, K, dk, midpt, K - 2, K + 1);
+            const SecSums s0 = E.sums(dk, midpt, K - 2, 2);
             r.iters++;
-            const double c = rhoinv + s0.psi;
+            const double c = rhoinv + s0.S;
             const double dl_ii = (dk1 - dk) - midpt, dl_n = -midpt;
-            const double w = c + zk1 * zk1 / dl_ii + zk * zk / dl_n;
+            const double w = c + zk12 / dl_ii + zk2 / dl_n;
             const double del = dk - dk1;
-            const double a = -c * del + zk1 * zk1 + zk * zk;
-            const double b = zk * zk * del;
+            const double a = -c * del + zk12 + zk2;
+            const double b = zk2 * del;
             if (w <= 0.0) {
-                const double temp = zk1 * zk1 / (dk - dk1 + rho) + zk * zk / rho;
+                const double temp = zk12 / (dk - dk1 + rho) + zk2 / rho;
                 if (c <= temp)
                     tau = rho;
                 else if (a < 0.0)
@@ -313,14 +308,15 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
         }
         double psi, dpsi, phi, dphi, erretm, w;
         auto eval = [&]() {
-            SecSums s = secular_sums(P, K, dk, tau, ii + 1, K - 1);
+            const SecSums s = E.sums(dk, tau, K - 1, 1);
             r.iters++;
-            psi = s.psi;
-            dpsi = s.dpsi;
-            phi = s.phi;
-            dphi = s.dphi;
-            erretm = fabs(s.err_psi);
-            erretm = 8.0 * (-phi - psi) + erretm - phi + rhoinv + fabs(tau) * (dpsi + dphi);
+            psi = s.S;
+            dpsi = s.dpsi + s.dphi;
+            const double err = s.J - (double)(K - 1) * s.S;
+            const double dn = -tau;
+            phi = zk2 / dn;
+            dphi = phi / dn;
+            erretm = 8.0 * (-phi - psi) + err - phi + rhoinv + fabs(tau) * (dpsi + dphi);
             w = rhoinv + phi + psi;
         };
         eval();
@@ -367,21 +363,21 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
 
     // ---- interior root, in (d_i, d_{i+1})
     const int ip1 = i + 1;
-    const double di = P.d(i), dip1 = P.d(ip1);
-    const double zi = P.z(i), zip1 = P.z(ip1);
+    const double di = E.d(i), dip1 = E.d(ip1);
+    const double zi2 = E.zsq(i), zip12 = E.zsq(ip1);
     const double del = dip1 - di;
     const double midpt = del / 2.0;
     bool orgati;
     double tau, dltlb, dltub;
     {
-        SecSums s0 = secular_sums(P, K, di, midpt, i, i + 2);
+        const SecSums s0 = E.sums(di, midpt, i, 2);
         r.iters++;
-        const double c = rhoinv + s0.psi + s0.phi;
-        const double w = c + zi * zi / (-midpt) + zip1 * zip1 / (del - midpt);
+        const double c = rhoinv + s0.S;
+        const double w = c + zi2 / (-midpt) + zip12 / (del - midpt);
         if (w > 0.0) {
             orgati = true;
-            const double a = c * del + zi * zi + zip1 * zip1;
-            const double b = zi * zi * del;
+            const double a = c * del + zi2 + zip12;
+            const double b = zi2 * del;
             if (a > 0.0)
                 tau = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
             else
@@ -390,8 +386,8 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
             dltub = midpt;
         } else {
             orgati = false;
-            const double a = c * del - zi * zi - zip1 * zip1;
-            const double b = zip1 * zip1 * del;
+            const double a = c * del - zi2 - zip12;
+            const double b = zip12 * del;
             if (a < 0.0)
                 tau = 2.0 * b / (a - sqrt(fabs(a * a + 4.0 * b * c)));
             else
@@ -403,30 +399,29 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
     const int ii = orgati ? i : ip1;
     const int iim1 = ii - 1, iip1 = ii + 1;
     const double dorg = orgati ? di : dip1;
-    const double zii = P.z(ii);
-    // neighbour poles used by the three-pole model (only valid when 0 < ii < K-1)
-    const double dm1 = (ii > 0) ? P.d(iim1) : 0.0;
-    const double zm1 = (ii > 0) ? P.z(iim1) : 0.0;
-    const double dp1 = (iip1 < K) ? P.d(iip1) : 0.0;
-    const double zp1 = (iip1 < K) ? P.z(iip1) : 0.0;
+    const double zii2 = E.zsq(ii);
+    // neighbour poles of the three-pole model (only used when 0 < ii < K-1)
+    const double dm1 = (ii > 0) ? E.d(iim1) : 0.0;
+    const double zm12 = (ii > 0) ? E.zsq(iim1) : 0.0;
+    const double dp1 = (iip1 < K) ? E.d(iip1) : 0.0;
+    const double zp12 = (iip1 < K) ? E.zsq(iip1) : 0.0;
 
-    double psi, dpsi, phi, dphi, erretm, w, w0, dw;
+    double A, S, dpsi, dphi, erretm, w, w0, dw;
     // w0: secular function without the origin pole's term; w: with it
     auto eval = [&](double tau_err) {
-        SecSums s = secular_sums(P, K, dorg, tau, ii, ii + 1);
+        const SecSums s = E.sums(dorg, tau, ii, 1);
         r.iters++;
-        psi = s.psi;
+        S = s.S;
+        A = s.A;
         dpsi = s.dpsi;
-        phi = s.phi;
         dphi = s.dphi;
-        erretm = fabs(s.err_psi) + s.err_phi;
+        const double err = s.J - (double)ii * s.S;
         const double dii = -tau;  // delta at the origin pole
-        double temp = zii / dii;
-        dw = dpsi + dphi + temp * temp;
-        temp = zii * temp;
-        w0 = rhoinv + phi + psi;
+        const double temp = zii2 / dii;
+        dw = dpsi + dphi + temp / dii;
+        w0 = rhoinv + S;
         w = w0 + temp;
-        erretm = 8.0 * (phi - psi) + erretm + 2.0 * rhoinv + 3.0 * fabs(temp) + tau_err * dw;
+        erretm = 8.0 * A + err + 2.0 * rhoinv + 3.0 * fabs(temp) + tau_err * dw;
     };
     eval(0.0);
     bool swtch3 = orgati ? (w0 < 0.0) : (w0 > 0.0);
@@ -460,15 +455,16 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
                 double c;
                 if (niter == 2 || !swtch) {
                     if (orgati)
-                        c = w - dl_ip1 * dw - (di - dip1) * (zi / dl_i) * (zi / dl_i);
+                        c = w - dl_ip1 * dw - (di - dip1) * (zi2 / (dl_i * dl_i));
                     else
-                        c = w - dl_i * dw - (dip1 - di) * (zip1 / dl_ip1) * (zip1 / dl_ip1);
+                        c = w - dl_i * dw - (dip1 - di) * (zip12 / (dl_ip1 * dl_ip1));
                 } else {
-                    const double temp = zii / delta_at(dorg);
+                    const double dlo = delta_at(dorg);
+                    const double temp = zii2 / (dlo * dlo);
                     if (orgati)
-                        dpsi += temp * temp;
+                        dpsi += temp;
                     else
-                        dphi += temp * temp;
+                        dphi += temp;
                     c = w - dl_i * dpsi - dl_ip1 * dphi;
                 }
                 double a = (dl_i + dl_ip1) * w - dl_i * dl_ip1 * dw;
@@ -477,9 +473,9 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
                     if (a == 0.0) {
                         if (niter == 2 || !swtch) {
                             if (orgati)
-                                a = zi * zi + dl_ip1 * dl_ip1 * (dpsi + dphi);
+                                a = zi2 + dl_ip1 * dl_ip1 * (dpsi + dphi);
                             else
-                                a = zip1 * zip1 + dl_i * dl_i * (dpsi + dphi);
+                                a = zip12 + dl_i * dl_i * (dpsi + dphi);
                         } else {
                             a = dl_i * dl_i * dpsi + dl_ip1 * dl_ip1 * dphi;
                         }
@@ -492,26 +488,24 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
                 }
             } else {
                 const double dl_m = delta_at(dm1), dl_p = delta_at(dp1);
-                const double temp = rhoinv + psi + phi;
+                const double temp = rhoinv + S;
                 double c, zz[3];
                 if (niter > 2 && swtch) {
                     c = temp - dl_m * dpsi - dl_p * dphi;
                     zz[0] = dl_m * dl_m * dpsi;
                     zz[2] = dl_p * dl_p * dphi;
                 } else if (orgati) {
-                    double t1 = zm1 / dl_m;
-                    t1 *= t1;
+                    const double t1 = zm12 / (dl_m * dl_m);
                     c = temp - dl_p * (dpsi + dphi) - (dm1 - dp1) * t1;
-                    zz[0] = zm1 * zm1;
+                    zz[0] = zm12;
                     zz[2] = dl_p * dl_p * ((dpsi - t1) + dphi);
                 } else {
-                    double t1 = zp1 / dl_p;
-                    t1 *= t1;
+                    const double t1 = zp12 / (dl_p * dl_p);
                     c = temp - dl_m * (dpsi + dphi) - (dp1 - dm1) * t1;
                     zz[0] = dl_m * dl_m * (dpsi + (dphi - t1));
-                    zz[2] = zp1 * zp1;
+                    zz[2] = zp12;
                 }
-                zz[1] = zii * zii;
+                zz[1] = zii2;
                 const double dd3[3] = {dl_m, delta_at(dorg), dl_p};
                 if (secular_three(niter, orgati, c, dd3, zz, w, &eta)) {
                     r.info = 1;
@@ -524,8 +518,7 @@ __host__ __device__ RootState secular_root(const Poles& P, int K, int i, double 
             if (temp > dltub || temp < dltlb)
                 eta = (w < 0.0) ? (dltub - tau) / 2.0 : (dltlb - tau) / 2.0;
             const double prew = w;
-            const double tau_new = tau + eta;
-            tau = tau_new;
+            tau += eta;
             eval(fabs(tau));
             if (niter == 2) {
                 swtch = orgati ? (-w > fabs(prew) / 10.0) : (w > fabs(prew) / 10.0);

@@ -4,11 +4,29 @@
 #include "../../ppopp27-brdc-ae_amd/csrc/brdc_secular.h"
 
 namespace {
-struct HostPoles {
+// Sequential evaluator: the same sums the device computes with L lanes per root.
+struct HostEval {
     const double* dp;
     const double* zp;
+    int K;
     double d(int j) const { return dp[j]; }
-    double z(int j) const { return zp[j]; }
+    double zsq(int j) const { return zp[j] * zp[j]; }
+    brdc::SecSums sums(double dorg, double tau, int ex_lo, int nex) const {
+        brdc::SecSums s = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < K; j++) {
+            if ((unsigned)(j - ex_lo) < (unsigned)nex) continue;
+            const double delta = (dp[j] - dorg) - tau;
+            const double r = brdc::rcp_nr(delta);
+            const double term = zsq(j) * r;
+            s.S += term;
+            s.A += fabs(term);
+            const double x = term * fabs(r);
+            if (x < 0.0) s.dpsi -= x;
+            else s.dphi += x;
+            s.J += (double)j * term;
+        }
+        return s;
+    }
 };
 }  // namespace
 
@@ -31,8 +49,8 @@ extern "C" int secular_host(int K, int i, const double* d, const double* z, doub
         out[5] = 1;
         return 0;
     }
-    HostPoles P{d, z};
-    const brdc::RootState r = brdc::secular_root(P, K, i, rho);
+    HostEval E{d, z, K};
+    const brdc::RootState r = brdc::secular_root(E, K, i, rho);
     out[0] = r.lambda;
     out[1] = r.tau;
     out[2] = r.near1;
