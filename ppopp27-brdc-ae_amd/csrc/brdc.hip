@@ -22,7 +22,7 @@ namespace brdc {
 namespace {
 
 constexpr int kDefaultLeaf = 32;
-constexpr int kWideGrid = 8;
+constexpr int kSecWaves = 4;  // waves per block of the secular kernel (independent waves)
 constexpr size_t kAlign = 256;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
@@ -35,9 +35,9 @@ struct Layout {
     size_t ad, az, adz, aq;
     size_t gd, gq1, gq2;
     size_t td, tq1, tq2;
-    size_t rlam, rtau, rn1, rn2, rdorg;
-    size_t w, qo1, qo2;
-    size_t rorg, P, rend, fl;
+    size_t rlam, rs, rn1, rn2;
+    size_t pw, qo1, qo2;
+    size_t P, rend, fl;
     size_t counts, info;
     size_t total;
     int levels;
@@ -78,14 +78,12 @@ Layout make_layout(int n, int leaf) {
     L.tq1 = take(nd);
     L.tq2 = take(nd);
     L.rlam = take(nd);
-    L.rtau = take(nd);
+    L.rs = take(2 * nd);
     L.rn1 = take(nd);
     L.rn2 = take(nd);
-    L.rdorg = take(nd);
-    L.w = take(nd);
+    L.pw = take(2 * nd);
     L.qo1 = take(nd);
     L.qo2 = take(nd);
-    L.rorg = take(ni);
     L.P = take(ni);
     L.rend = take(ni);
     L.fl = take(ni);
@@ -144,6 +142,18 @@ struct ProfScope {
         g_prof.marks.push_back({cls, {a, b}});
     }
 };
+
+// the O(K^2) kernels come in widths of 1..16 waves per 64-item group
+template <class K1, class K2, class K4, class K8, class... Args>
+void launch_wide(K1 k1, K2 k2, K4 k4, K8 k8, int WL, int blocks, hipStream_t st, Args... args) {
+    const dim3 grid(blocks);
+    switch (WL) {
+        case 1: hipLaunchKernelGGL(k1, grid, dim3(kGroup), 0, st, args...); break;
+        case 2: hipLaunchKernelGGL(k2, grid, dim3(2 * kGroup), 0, st, args...); break;
+        case 4: hipLaunchKernelGGL(k4, grid, dim3(4 * kGroup), 0, st, args...); break;
+        default: hipLaunchKernelGGL(k8, grid, dim3(8 * kGroup), 0, st, args...); break;
+    }
+}
 
 // deflation block size follows the level's node size
 template <class... Args>
@@ -207,14 +217,12 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     double* tq1 = at<double>(ws, L.tq1);
     double* tq2 = at<double>(ws, L.tq2);
     double* rlam = at<double>(ws, L.rlam);
-    double* rtau = at<double>(ws, L.rtau);
+    double2* rs = at<double2>(ws, L.rs);
     double* rn1 = at<double>(ws, L.rn1);
     double* rn2 = at<double>(ws, L.rn2);
-    double* rdorg = at<double>(ws, L.rdorg);
-    double* w = at<double>(ws, L.w);
+    double2* pw = at<double2>(ws, L.pw);
     double* qo1 = at<double>(ws, L.qo1);
     double* qo2 = at<double>(ws, L.qo2);
-    int* rorg = at<int>(ws, L.rorg);
     int* P = at<int>(ws, L.P);
     int* rend = at<int>(ws, L.rend);
     int* fl = at<int>(ws, L.fl);
@@ -224,10 +232,9 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         const int maxsz = max_node_size(n, depth);
         const int bpm = (maxsz + kThreads - 1) / kThreads;
         const dim3 grid(merges * bpm);
-        // the O(K^2) kernels get kWideGrid threads per row so that several
-        // lanes can share one root or pole (lanes_per_item)
-        const int bpmw = bpm * kWideGrid;
-        const dim3 gridw(merges * bpmw);
+        // the O(K^2) kernels: one block per 64 roots/poles, WL waves wide
+        const int bpg = (maxsz + kGroup - 1) / kGroup;
+        const int WL = waves_for(maxsz, kMaxWaves);
         int* counts = counts_all + 4 * (merges - 1);
         const int nxt = cur ^ 1;
         const int root = (depth == 0);
@@ -246,21 +253,29 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         BRDC_LAUNCH_CHECK();
         {
             ProfScope ps(kSecular, st);
-            hipLaunchKernelGGL(k_secular, gridw, dim3(kThreads), 0, st, n, depth, bpmw, ew, counts,
-                               ad, az, adz, rlam, rtau, rn1, rn2, rdorg, rorg, counts, info);
+            // waves needed per merge: K * L / 64 <= maxsz roots' worth at L = 64
+            const int bpm_sec = (maxsz + kSecWaves - 1) / kSecWaves;
+            hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(merges * bpm_sec), dim3(kWave * kSecWaves),
+                               0, st, n, depth, bpm_sec, (const double*)ew, (const int*)counts,
+                               (const double*)ad, (const double*)az, (const double2*)adz, rlam,
+                               rs, rn1, rn2, counts, info);
         }
         BRDC_LAUNCH_CHECK();
         if (!root) {
             {
                 ProfScope ps(kWprod, st);
-                hipLaunchKernelGGL(k_wprod, gridw, dim3(kThreads), 0, st, n, depth, bpmw, counts,
-                                   ad, az, rtau, rn1, rn2, rdorg, w);
+                launch_wide(k_wprod<1>, k_wprod<2>, k_wprod<4>, k_wprod<8>, WL,
+                            merges * bpg, st, n, depth, bpg, (const int*)counts,
+                            (const double*)ad, (const double*)az, (const double2*)rs,
+                            (const double*)rn1, (const double*)rn2, pw);
             }
             BRDC_LAUNCH_CHECK();
             {
                 ProfScope ps(kRows, st);
-                hipLaunchKernelGGL(k_rows, gridw, dim3(kThreads), 0, st, n, depth, bpmw, counts,
-                                   ad, aq, w, rtau, rn1, rn2, rdorg, qo1, qo2);
+                launch_wide(k_rows<1>, k_rows<2>, k_rows<4>, k_rows<8>, WL,
+                            merges * bpg, st, n, depth, bpg, (const int*)counts,
+                            (const double2*)aq, (const double2*)pw, (const double2*)rs,
+                            (const double*)rn1, (const double*)rn2, qo1, qo2);
             }
             BRDC_LAUNCH_CHECK();
         }

@@ -401,6 +401,118 @@ __device__ __forceinline__ bool givens_test(double dl, double zl, double dj, dou
     return fabs(t * c * sn) <= tol;
 }
 
+// wave-level scans (wave64)
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(v, o, kWave);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_incl_max(int v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(v, o, kWave);
+        if (lane >= o) v = max(v, y);
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Replays the real Givens run starting at t (ending before e): writes the
+// deflated values and rows at g[goff..), keeps them ascending within the run,
+// and stores the carried state of the run's surviving pole P[e-1].
+__device__ __forceinline__ void replay_run(int s, int t, int e, int goff, double tol,
+                                           const int* __restrict__ P, double* __restrict__ md,
+                                           double* __restrict__ mz, double* __restrict__ mq1,
+                                           double* __restrict__ mq2, int* __restrict__ fl,
+                                           double* __restrict__ gd, double* __restrict__ gq1,
+                                           double* __restrict__ gq2) {
+    int cur = P[s + t - 1];
+    double cd = md[s + cur], cz = mz[s + cur], c1 = mq1[s + cur], c2 = mq2[s + cur];
+    // every value the run produces is a convex combination of the run's
+    // original poles; clamping to their range removes last-ulp excursions so
+    // the three output lists stay exactly sorted
+    const double lo = cd;
+    double hi = cd;
+    const int base = goff;
+    for (int u = t; u < e; u++) {
+        const int j = P[s + u];
+        const double dj = md[s + j], qj1 = mq1[s + j], qj2 = mq2[s + j];
+        hi = dj;
+        GivensStep g;
+        givens_test(cd, cz, dj, mz[s + j], tol, &g);
+        // DROT on the two selected rows (dlaed7_br.f:323-325) and the D
+        // update (dlaed7_br.f:326-328)
+        gd[s + goff] = fmin(fmax(cd * g.c * g.c + dj * g.s * g.s, lo), dj);
+        gq1[s + goff] = g.c * c1 + g.s * qj1;
+        gq2[s + goff] = g.c * c2 + g.s * qj2;
+        goff++;
+        fl[s + cur] = 1;
+        const double nd = fmin(fmax(cd * g.s * g.s + dj * g.c * g.c, lo), dj);
+        const double n1 = g.c * qj1 - g.s * c1;
+        const double n2 = g.c * qj2 - g.s * c2;
+        cd = nd;
+        cz = g.tau;
+        c1 = n1;
+        c2 = n2;
+        cur = j;
+    }
+    md[s + cur] = fmin(fmax(cd, lo), hi);
+    mz[s + cur] = cz;
+    mq1[s + cur] = c1;
+    mq2[s + cur] = c2;
+    for (int a = base + 1; a < goff; a++) {
+        const double x = gd[s + a], x1 = gq1[s + a], x2 = gq2[s + a];
+        int b = a - 1;
+        while (b >= base && gd[s + b] > x) {
+            gd[s + b + 1] = gd[s + b];
+            gq1[s + b + 1] = gq1[s + b];
+            gq2[s + b + 1] = gq2[s + b];
+            b--;
+        }
+        gd[s + b + 1] = x;
+        gq1[s + b + 1] = x1;
+        gq2[s + b + 1] = x2;
+    }
+}
+
+// speculative end of the Givens run that starts at t if P[t-1] is unrotated:
+// the first u whose test fails on the carried state, m if none does, or -2
+// when the walk exceeds cap steps; -1 if pair (P[t-1], P[t]) does not rotate
+__device__ __forceinline__ int speculative_end(int s, int t, int m, int cap, double tol,
+                                               const int* __restrict__ P,
+                                               const double* __restrict__ md,
+                                               const double* __restrict__ mz) {
+    const int jl = P[s + t - 1], j = P[s + t];
+    GivensStep g;
+    const double dj = md[s + j];
+    if (!givens_test(md[s + jl], mz[s + jl], dj, mz[s + j], tol, &g)) return -1;
+    double cd = md[s + jl] * g.s * g.s + dj * g.c * g.c;
+    double cz = g.tau;
+    int u = t + 1, steps = 0;
+    while (u < m && steps < cap) {
+        const int jj = P[s + u];
+        const double d2 = md[s + jj];
+        if (!givens_test(cd, cz, d2, mz[s + jj], tol, &g)) break;
+        cd = cd * g.s * g.s + d2 * g.c * g.c;
+        cz = g.tau;
+        u++;
+        steps++;
+    }
+    return (u < m && steps == cap) ? -2 : u;
+}
+
+// One block per merge; each wave owns a contiguous range of positions (and
+// later of t), walked in coalesced 64-wide tiles with ballot/popcount
+// compaction; cross-wave offsets come from per-wave totals in LDS.
 template <int T>
 __global__ __launch_bounds__(T) void k_deflate(
     int n, int depth, const double* __restrict__ ew, double* __restrict__ md,
@@ -409,70 +521,88 @@ __global__ __launch_bounds__(T) void k_deflate(
     double2* __restrict__ adz, double2* __restrict__ aq, double* __restrict__ gd,
     double* __restrict__ gq1, double* __restrict__ gq2, double* __restrict__ td,
     double* __restrict__ tq1, double* __restrict__ tq2, int* __restrict__ counts) {
-    __shared__ int lds[T / kWave + 2];
-    __shared__ int any_fix;
+    constexpr int NW = T / kWave;
+    __shared__ int wa[NW], wb[NW];
+    __shared__ int s_fix;
     const int merge = blockIdx.x;
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
     const int cut = sz / 2;
     const double rho = fabs(2.0 * ew[s + cut - 1]);
     const double tol = 8.0 * kEps * fmax(fabs(md[s]), fabs(md[s + sz - 1]));
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    const unsigned long long lt = (1ull << lane) - 1ull;
 
-    // ---- A: tiny-z flags, non-tiny compaction
-    const int seg = (sz + T - 1) / T;
-    const int p0 = min(sz, tid * seg), p1 = min(sz, p0 + seg);
+    // ---- A: tiny-z flags, non-tiny compaction into P[0..m)
+    const int R = ((sz + NW - 1) / NW + kWave - 1) / kWave * kWave;
+    const int pb = min(sz, w * R), pe = min(sz, pb + R);
     int cnt = 0;
-#pragma unroll 8
-    for (int p = p0; p < p1; p++) {
-        const bool keep = rho * fabs(mz[s + p]) > tol;
-        cnt += keep;
-        fl[s + p] = keep ? 0 : 2;
+    for (int base = pb; base < pe; base += kWave) {
+        const int p = base + lane;
+        bool keep = false;
+        if (p < pe) {
+            keep = rho * fabs(mz[s + p]) > tol;
+            fl[s + p] = keep ? 0 : 2;
+        }
+        cnt += __popcll(__ballot(keep));
     }
-    int m;
-    int off = block_exclusive_sum<T>(cnt, &m, lds);
-    for (int p = p0; p < p1; p++)
-        if (fl[s + p] == 0) P[s + off++] = p;
-    if (tid == 0) any_fix = 0;
+    if (lane == 0) wa[w] = cnt;
+    if (tid == 0) s_fix = 0;
+    __syncthreads();
+    int off = 0, m = 0;
+    for (int q = 0; q < NW; q++) {
+        const int v = wa[q];
+        if (q < w) off += v;
+        m += v;
+    }
+    for (int base = pb; base < pe; base += kWave) {
+        const int p = base + lane;
+        const bool keep = (p < pe) && fl[s + p] == 0;
+        const unsigned long long b = __ballot(keep);
+        if (keep) P[s + off + __popcll(b & lt)] = p;
+        off += __popcll(b);
+    }
     __syncthreads();
 
-    // ---- B: speculative runs from every rotating pair
-    const int tseg = (m + T - 1) / T;
-    const int t1 = min(m, (tid + 1) * tseg);
-    const int t0 = max(1, min(m, tid * tseg));
-    int first_cand = INT_MAX, max_end = INT_MIN;
-    bool unknown = false;
-    for (int t = t0; t < t1; t++) {
-        const int jl = P[s + t - 1], j = P[s + t];
-        GivensStep g;
-        const double dj = md[s + j];
-        if (givens_test(md[s + jl], mz[s + jl], dj, mz[s + j], tol, &g)) {
-            double cd = md[s + jl] * g.s * g.s + dj * g.c * g.c;
-            double cz = g.tau;
-            int u = t + 1, steps = 0;
-            while (u < m && steps < kSpecCap) {
-                const int jj = P[s + u];
-                const double d2 = md[s + jj];
-                if (!givens_test(cd, cz, d2, mz[s + jj], tol, &g)) break;
-                cd = cd * g.s * g.s + d2 * g.c * g.c;
-                cz = g.tau;
-                u++;
-                steps++;
-            }
-            const int e = (u < m && steps == kSpecCap) ? -2 : u;
-            rend[s + t] = e;
-            if (first_cand == INT_MAX) first_cand = t;
-            if (e == -2) unknown = true;
-            else max_end = max(max_end, e);
-        } else {
-            rend[s + t] = -1;
-        }
-    }
-    const int carry = block_exclusive_max<T>(unknown ? INT_MAX : max_end, lds);
-    if (unknown || (first_cand != INT_MAX && carry >= first_cand)) any_fix = 1;
+    // ---- B: speculative runs from every rotating pair (independent per t)
+    for (int t = 1 + tid; t < m; t += T) rend[s + t] = speculative_end(s, t, m, kSpecCap, tol, P, md, mz);
     __syncthreads();
-    if (any_fix) {
-        // exact sequential resolution (rare: overlapping or very long runs)
+
+    // ---- B2: disjoint speculative runs make every candidate a real start
+    // (the sequential walk would start each of them); any overlap, or a walk
+    // past the cap, goes through one exact sequential resolution instead
+    const int Rt = ((m + NW - 1) / NW + kWave - 1) / kWave * kWave;
+    const int tb = min(m, w * Rt), te = min(m, tb + Rt);
+    int carry = INT_MIN, first = INT_MAX;
+    bool bad = false;
+    for (int base = tb; base < te; base += kWave) {
+        const int t = base + lane;
+        const int e = (t >= 1 && t < te) ? rend[s + t] : -1;
+        if (e == -2) bad = true;
+        const int incl = wave_incl_max(e == -2 ? INT_MAX : e);
+        int excl = __shfl_up(incl, 1, kWave);
+        if (lane == 0) excl = INT_MIN;
+        excl = max(excl, carry);
+        if (e >= 0 && excl >= t) bad = true;
+        if (e >= 0) first = min(first, t);
+        carry = max(carry, __shfl(incl, kWave - 1, kWave));
+    }
+    first = wave_min(first);
+    const bool wbad = __any(bad);
+    __syncthreads();  // wa/wb reuse
+    if (lane == 0) {
+        wa[w] = carry;
+        wb[w] = first;
+        if (wbad) s_fix = 1;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int cin = INT_MIN;
+        for (int q = 0; q < w; q++) cin = max(cin, wa[q]);
+        if (wb[w] != INT_MAX && cin >= wb[w]) s_fix = 1;
+    }
+    __syncthreads();
+    if (s_fix) {
         if (tid == 0) {
             int cover = -1;
             for (int t = 1; t < m; t++) {
@@ -483,125 +613,85 @@ __global__ __launch_bounds__(T) void k_deflate(
                     continue;
                 }
                 if (e == -2) {  // finish the capped walk
-                    const int jl = P[s + t - 1], j = P[s + t];
-                    GivensStep g;
-                    givens_test(md[s + jl], mz[s + jl], md[s + j], mz[s + j], tol, &g);
-                    double cd = md[s + jl] * g.s * g.s + md[s + j] * g.c * g.c;
-                    double cz = g.tau;
-                    int u = t + 1;
-                    while (u < m) {
-                        const int jj = P[s + u];
-                        const double d2 = md[s + jj];
-                        if (!givens_test(cd, cz, d2, mz[s + jj], tol, &g)) break;
-                        cd = cd * g.s * g.s + d2 * g.c * g.c;
-                        cz = g.tau;
-                        u++;
-                    }
-                    e = u;
+                    e = speculative_end(s, t, m, m, tol, P, md, mz);
                     rend[s + t] = e;
                 }
                 cover = e;
             }
         }
-    } else {
-        int cover = -1;
-        for (int t = t0; t < t1; t++) {
-            const int e = rend[s + t];
-            if (e == -1) continue;
-            if (t <= cover) rend[s + t] = -1;
-            else cover = e;
-        }
+        __syncthreads();
+    }
+
+    // ---- C: Givens-list offsets (scan of run lengths over t) and replay
+    int runs = 0;
+    for (int base = tb; base < te; base += kWave) {
+        const int t = base + lane;
+        const int e = (t >= 1 && t < te) ? rend[s + t] : -1;
+        runs += (e >= 0) ? e - t : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) runs += __shfl_xor(runs, o, kWave);
+    __syncthreads();  // wa reuse
+    if (lane == 0) wa[w] = runs;
+    __syncthreads();
+    int goff = 0, G = 0;
+    for (int q = 0; q < NW; q++) {
+        const int v = wa[q];
+        if (q < w) goff += v;
+        G += v;
+    }
+    for (int base = tb; base < te; base += kWave) {
+        const int t = base + lane;
+        const int e = (t >= 1 && t < te) ? rend[s + t] : -1;
+        const int len = (e >= 0) ? e - t : 0;
+        const int incl = wave_incl_sum(len);
+        if (e >= 0) replay_run(s, t, e, goff + incl - len, tol, P, md, mz, mq1, mq2, fl, gd, gq1, gq2);
+        goff += __shfl(incl, kWave - 1, kWave);
     }
     __syncthreads();
 
-    // ---- C: replay real runs, write deflated values and carried states
-    int my_g = 0;
-    for (int t = t0; t < t1; t++) {
-        const int e = rend[s + t];
-        if (e >= 0) my_g += e - t;
-    }
-    int G;
-    int goff = block_exclusive_sum<T>(my_g, &G, lds);
-    for (int t = t0; t < t1; t++) {
-        const int e = rend[s + t];
-        if (e < 0) continue;
-        int cur = P[s + t - 1];
-        double cd = md[s + cur], cz = mz[s + cur], c1 = mq1[s + cur], c2 = mq2[s + cur];
-        // every value the run produces is a convex combination of the run's
-        // original poles; clamping to their range removes last-ulp excursions
-        // so the three output lists stay exactly sorted
-        const double lo = cd;
-        double hi = cd;
-        const int base = goff;
-        for (int u = t; u < e; u++) {
-            const int j = P[s + u];
-            const double dj = md[s + j], qj1 = mq1[s + j], qj2 = mq2[s + j];
-            hi = dj;
-            GivensStep g;
-            givens_test(cd, cz, dj, mz[s + j], tol, &g);
-            // DROT on the two selected rows (dlaed7_br.f:323-325) and the
-            // D update (dlaed7_br.f:326-328)
-            gd[s + goff] = fmin(fmax(cd * g.c * g.c + dj * g.s * g.s, lo), dj);
-            gq1[s + goff] = g.c * c1 + g.s * qj1;
-            gq2[s + goff] = g.c * c2 + g.s * qj2;
-            goff++;
-            fl[s + cur] = 1;
-            const double nd = fmin(fmax(cd * g.s * g.s + dj * g.c * g.c, lo), dj);
-            const double n1 = g.c * qj1 - g.s * c1;
-            const double n2 = g.c * qj2 - g.s * c2;
-            cd = nd;
-            cz = g.tau;
-            c1 = n1;
-            c2 = n2;
-            cur = j;
-        }
-        md[s + cur] = fmin(fmax(cd, lo), hi);
-        mz[s + cur] = cz;
-        mq1[s + cur] = c1;
-        mq2[s + cur] = c2;
-        // keep this run's deflated values ascending
-        for (int a = base + 1; a < goff; a++) {
-            const double x = gd[s + a], x1 = gq1[s + a], x2 = gq2[s + a];
-            int b = a - 1;
-            while (b >= base && gd[s + b] > x) {
-                gd[s + b + 1] = gd[s + b];
-                gq1[s + b + 1] = gq1[s + b];
-                gq2[s + b + 1] = gq2[s + b];
-                b--;
-            }
-            gd[s + b + 1] = x;
-            gq1[s + b + 1] = x1;
-            gq2[s + b + 1] = x2;
-        }
-    }
-    __syncthreads();
-
-    // ---- D: compaction of kept poles and tiny-deflated entries
+    // ---- D: compaction into kept poles and tiny-deflated entries
     int na = 0, nt = 0;
-#pragma unroll 8
-    for (int p = p0; p < p1; p++) {
-        const int f = fl[s + p];
-        na += (f == 0);
-        nt += (f == 2);
+    for (int base = pb; base < pe; base += kWave) {
+        const int p = base + lane;
+        const int f = (p < pe) ? fl[s + p] : 1;
+        na += __popcll(__ballot(f == 0));
+        nt += __popcll(__ballot(f == 2));
     }
-    int K, Tn;
-    int aoff = block_exclusive_sum<T>(na, &K, lds);
-    int toff = block_exclusive_sum<T>(nt, &Tn, lds);
-    for (int p = p0; p < p1; p++) {
-        const int f = fl[s + p];
-        if (f == 0) {
-            const double dv = md[s + p], zv = mz[s + p];
-            ad[s + aoff] = dv;
-            az[s + aoff] = zv;
-            adz[s + aoff] = make_double2(dv, zv * zv);
-            aq[s + aoff] = make_double2(mq1[s + p], mq2[s + p]);
-            aoff++;
-        } else if (f == 2) {
-            td[s + toff] = md[s + p];
-            tq1[s + toff] = mq1[s + p];
-            tq2[s + toff] = mq2[s + p];
-            toff++;
+    __syncthreads();  // wa/wb reuse
+    if (lane == 0) {
+        wa[w] = na;
+        wb[w] = nt;
+    }
+    __syncthreads();
+    int aoff = 0, toff = 0, K = 0, Tn = 0;
+    for (int q = 0; q < NW; q++) {
+        const int va = wa[q], vt = wb[q];
+        if (q < w) {
+            aoff += va;
+            toff += vt;
         }
+        K += va;
+        Tn += vt;
+    }
+    for (int base = pb; base < pe; base += kWave) {
+        const int p = base + lane;
+        const int f = (p < pe) ? fl[s + p] : 1;
+        const unsigned long long ba = __ballot(f == 0), bt = __ballot(f == 2);
+        if (f == 0) {
+            const int o = s + aoff + __popcll(ba & lt);
+            const double dv = md[s + p], zv = mz[s + p];
+            ad[o] = dv;
+            az[o] = zv;
+            adz[o] = make_double2(dv, zv * zv);
+            aq[o] = make_double2(mq1[s + p], mq2[s + p]);
+        } else if (f == 2) {
+            const int o = s + toff + __popcll(bt & lt);
+            td[o] = md[s + p];
+            tq1[o] = mq1[s + p];
+            tq2[o] = mq2[s + p];
+        }
+        aoff += __popcll(ba);
+        toff += __popcll(bt);
     }
     if (tid == 0) {
         counts[4 * merge + 0] = K;
@@ -612,169 +702,266 @@ __global__ __launch_bounds__(T) void k_deflate(
 }
 
 // ---------------------------------------------------------------------------
-// Merge steps 3-5 share one work layout: L lanes per item (root or pole),
-// L a power of two chosen from the merge's K at run time so the level's grid
-// is filled and each lane streams about K/L poles or roots.  Lane c of an
-// item takes the strided indices c, c+L, ... (coalesced across the L lanes);
-// partial sums meet in an xor butterfly inside the item's lane group.
-__device__ __forceinline__ int lanes_per_item(int K, int capacity) {
-    int L = kWave;
-    while (L > 1 && (K * L > capacity || K < 16 * L)) L >>= 1;
-    return L;
-}
+// Merge steps 3-5 share one work layout.  A block owns 64 consecutive items
+// (roots or poles) of one merge, one item per lane; every lane of a wave
+// streams the same index of the other dimension at the same time, so the
+// streamed data (pole or root state) is wave-uniform and comes in through
+// scalar loads.  For large K the block's W waves split the stream into W
+// contiguous chunks and their partial results meet once per pass in LDS.
+// W follows K at run time (at most the launch's width WL): about kChunk or
+// more streamed entries per wave.
+constexpr int kGroup = 64;
+constexpr int kChunk = 256;
 
-template <class T>
-__device__ __forceinline__ T group_sum(T v, int L) {
-    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+// Streamed data is read through the constant address space with wave-uniform
+// indices, which selects scalar (SMEM) loads into SGPRs.
+using cdbl = const __attribute__((address_space(4))) double*;
+struct CPairs {  // array of (x, y) double pairs viewed through the constant space
+    cdbl p;
+    __device__ __forceinline__ double2 operator[](int j) const {
+        return make_double2(p[2 * j], p[2 * j + 1]);
+    }
+};
+struct CDoubles {
+    cdbl p;
+    __device__ __forceinline__ double operator[](int j) const { return p[j]; }
+};
+__device__ __forceinline__ CPairs as_const(const double2* q) {
+    return CPairs{(cdbl) reinterpret_cast<const double*>(q)};
+}
+__device__ __forceinline__ CDoubles as_const(const double* q) { return CDoubles{(cdbl)q}; }
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+constexpr int kMaxWaves = 8;  // 512 threads: the root solver needs ~180 VGPRs per lane
+
+__host__ __device__ inline int waves_for(int K, int Wmax) {
+    int W = 1;
+    while (W < Wmax && K >= 2 * W * kChunk) W <<= 1;
+    return W;
 }
 
 // Merge step 3: secular roots (DLAED9_BR's DLAED4_BR loop, dlaed9_br.f:181-262);
 // per root the compact state (org, tau, near1, near2) plus d[org].
-struct LaneEval {
+//
+// Layout: L lanes per root (L a power of two picked per merge from K and the
+// level's merge count), 64/L roots per wavefront, waves independent (no
+// barriers).  Lane c of a root sums the poles c, c+L, c+2L, ... (loads
+// coalesced across the root's L lanes, shared by the wave's roots), and a
+// root's lanes combine their sums in an xor butterfly.  A wave therefore
+// runs only as many passes as its 64/L roots need, and a small merge is
+// spread over many lanes for latency.
+struct PoleView {
     const double2* __restrict__ dz;  // (d_j, z_j^2)
-    int K, L, c;
     __device__ __forceinline__ double d(int j) const { return dz[j].x; }
     __device__ __forceinline__ double zsq(int j) const { return dz[j].y; }
-    __device__ SecSums sums(double dorg, double tau, int ex_lo, int nex) const {
-        double S = 0.0, A = 0.0, DP = 0.0, DF = 0.0, J = 0.0;
-        double jd = (double)c;
-        const double dL = (double)L;
-#pragma unroll 2
-        for (int j = c; j < K; j += L) {
-            const double2 p = dz[j];
-            const double delta = (p.x - dorg) - tau;
-            const double r = rcp_nr(delta);
-            double term = p.y * r;
-            if ((unsigned)(j - ex_lo) < (unsigned)nex) term = 0.0;
-            S += term;
-            A += fabs(term);
-            const double x = term * fabs(r);
-            DP += fmin(x, 0.0);
-            DF += fmax(x, 0.0);
-            J = fma(jd, term, J);
-            jd += dL;
-        }
-        SecSums out;
-        out.S = group_sum(S, L);
-        out.A = group_sum(A, L);
-        out.dpsi = -group_sum(DP, L);
-        out.dphi = group_sum(DF, L);
-        out.J = group_sum(J, L);
-        return out;
-    }
 };
 
-__global__ __launch_bounds__(kThreads) void k_secular(
+constexpr int kBatch = 8;
+
+__device__ __forceinline__ void secular_term(double2 p, int j, double dorg, double tau, int exlo,
+                                             int nex, double acc[5]) {
+    const double delta = (p.x - dorg) - tau;
+    const double r = rcp_nr(delta);
+    double term = p.y * r;
+    if ((unsigned)(j - exlo) < (unsigned)nex) term = 0.0;
+    acc[0] += term;
+    acc[1] += fabs(term);
+    const double x = term * fabs(r);
+    acc[2] += fmin(x, 0.0);
+    acc[3] += fmax(x, 0.0);
+    acc[4] = fma((double)j, term, acc[4]);
+}
+
+// lanes per root: about 4+ poles per lane per pass, and no more lanes than
+// the level needs to fill the machine (~kTargetLanes lanes in flight)
+constexpr int kTargetLanes = 1 << 19;
+__host__ __device__ inline int secular_lanes(int K, int merges) {
+    int L = 1;
+    while (L < kWave && 4 * 2 * L <= K && (long long)merges * K * (2 * L) <= kTargetLanes) L <<= 1;
+    return L;
+}
+
+template <int WL>
+__global__ __launch_bounds__(kWave * WL) void k_secular(
     int n, int depth, int bpm, const double* __restrict__ ew, const int* counts,
     const double* __restrict__ ad, const double* __restrict__ az,
-    const double2* __restrict__ adz, double* __restrict__ rlam, double* __restrict__ rtau,
-    double* __restrict__ rn1, double* __restrict__ rn2, double* __restrict__ rdorg,
-    int* __restrict__ rorg, int* counts_rw, int* info) {
+    const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
+    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    const int K = counts[4 * merge];
-    const int L = lanes_per_item(K, bpm * kThreads);
-    const int per_block = kThreads / L;
-    if (part * per_block >= K) return;
+    const int K = uniform(counts[4 * merge]);
+    const int L = secular_lanes(K, 1 << depth);
+    const int rpw = kWave / L;  // roots per wave
+    const int wave = part * WL + (int)(threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = wave * rpw + lane / L;  // this lane's root
+    const int c = lane & (L - 1);
+    if (wave * rpw >= K) return;  // whole wave idle
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
     const int cut = sz / 2;
     const double rho = fabs(2.0 * ew[s + cut - 1]);
-    const int c = threadIdx.x & (L - 1);
-    const int j = part * per_block + threadIdx.x / L;
-    int iters = 0;
-    if (j < K) {
+    if (K <= 2) {
+        if (i >= K || c != 0) return;
         if (K == 1) {
             const double z0 = az[s];
             rlam[s] = ad[s] + rho * z0 * z0;
-            rtau[s] = rho * z0 * z0;
+            rs[s] = make_double2(ad[s], rho * z0 * z0);
             rn1[s] = 1.0;  // secular eigenvector (1)
             rn2[s] = 0.0;
-            rorg[s] = 0;
-            rdorg[s] = ad[s];
-            iters = 1;
-        } else if (K == 2) {
-            double lam, v0, v1, tau;
-            secular_two(j, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
-            rlam[s + j] = lam;
-            rtau[s + j] = tau;
-            rn1[s + j] = v0;  // secular eigenvector (v0, v1)
-            rn2[s + j] = v1;
-            rorg[s + j] = 0;
-            rdorg[s + j] = ad[s];
-            iters = 1;
         } else {
-            const LaneEval E{adz + s, K, L, c};
-            const RootState r = secular_root(E, K, j, rho);
-            if (c == 0) {
-                rlam[s + j] = r.lambda;
-                rtau[s + j] = r.tau;
-                rn1[s + j] = r.near1;
-                rn2[s + j] = r.near2;
-                rorg[s + j] = r.org;
-                rdorg[s + j] = ad[s + r.org];
-                iters = r.iters;
-                if (r.info) report_failure(info, n, s, sz);
-            }
+            double lam, v0, v1, tau;
+            secular_two(i, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
+            rlam[s + i] = lam;
+            rs[s + i] = make_double2(ad[s], tau);
+            rn1[s + i] = v0;  // secular eigenvector (v0, v1)
+            rn2[s + i] = v1;
         }
+        atomicAdd(&counts_rw[4 * merge + 3], 1);
+        return;
+    }
+    const double2* __restrict__ dz = adz + s;
+    const PoleView E{dz};
+    SecularSolve<PoleView> st;
+    bool active = (i < K);
+    st.init(E, K, min(i, K - 1), rho);
+    for (;;) {
+        if (!__any(active)) break;
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        int j = c;
+        for (; j + (kBatch - 1) * L < K; j += kBatch * L) {
+            double2 p[kBatch];
+#pragma unroll
+            for (int b = 0; b < kBatch; b++) p[b] = dz[j + b * L];
+#pragma unroll
+            for (int b = 0; b < kBatch; b++)
+                secular_term(p[b], j + b * L, st.ev_org, st.ev_tau, st.ev_exlo, st.ev_nex, acc);
+        }
+        for (; j < K; j += L)
+            secular_term(dz[j], j, st.ev_org, st.ev_tau, st.ev_exlo, st.ev_nex, acc);
+        for (int o = 1; o < L; o <<= 1) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+        }
+        SecSums sums;
+        sums.S = acc[0];
+        sums.A = acc[1];
+        sums.dpsi = -acc[2];
+        sums.dphi = acc[3];
+        sums.J = acc[4];
+        if (active && st.advance(E, sums)) active = false;
+    }
+    int iters = 0;
+    if (i < K && c == 0) {
+        const RootState& r = st.out;
+        rlam[s + i] = r.lambda;
+        rs[s + i] = make_double2(dz[r.org].x, r.tau);
+        rn1[s + i] = r.near1;
+        rn2[s + i] = r.near2;
+        iters = r.iters;
+        if (r.info) report_failure(info, n, s, sz);
     }
     for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
-    if ((threadIdx.x & 63) == 0 && iters) atomicAdd(&counts_rw[4 * merge + 3], iters);
+    if (lane == 0 && iters) atomicAdd(&counts_rw[4 * merge + 3], iters);
 }
 
-// Merge step 4: product-form update vector (dlaed9_br.f:218-266): L lanes per
+// Merge step 4: product-form update vector (dlaed9_br.f:218-266): one lane per
 // pole i streaming the roots j: w_i = sign(z_i) sqrt(-prod_j delta_j(i) /
 // prod_{j != i} (d_i - d_j)).  delta_j(i) comes from root j's compact state;
 // the two pole-adjacent entries are root j's near1/near2 (dlaed9_br.f:272-331).
-__global__ __launch_bounds__(kThreads) void k_wprod(
-    int n, int depth, int bpm, const int* __restrict__ counts, const double* __restrict__ ad,
-    const double* __restrict__ az, const double* __restrict__ rtau,
-    const double* __restrict__ rn1, const double* __restrict__ rn2,
-    const double* __restrict__ rdorg, double* __restrict__ w) {
-    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    const int K = counts[4 * merge];
-    if (K <= 2) return;
-    const int L = lanes_per_item(K, bpm * kThreads);
-    const int per_block = kThreads / L;
-    if (part * per_block >= K) return;
+template <int WL>
+__global__ __launch_bounds__(kGroup * WL) void k_wprod(
+    int n, int depth, int bpg, const int* __restrict__ counts, const double* __restrict__ ad,
+    const double* __restrict__ az, const double2* __restrict__ rs,
+    const double* __restrict__ rn1, const double* __restrict__ rn2, double2* __restrict__ pw) {
+    __shared__ double red[WL > 1 ? WL : 1][kGroup];
+    const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
+    const int K = uniform(counts[4 * merge]);
+    const int g0 = g * kGroup;
+    if (K <= 2 || g0 >= K) return;
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
-    const int c = threadIdx.x & (L - 1);
-    const int i = part * per_block + threadIdx.x / L;
-    if (i >= K) return;  // whole lane groups leave together
-    const double* __restrict__ d = ad + s;
-    const double di = d[i];
+    const int lane = threadIdx.x & (kGroup - 1);
+    const int c = uniform(threadIdx.x / kGroup);
+    const int W = waves_for(K, WL);
+    if (W == 1 && c > 0) return;
+    const int j0 = (c < W) ? (int)(((long long)K * c) / W) : K;
+    const int j1 = (c < W) ? (int)(((long long)K * (c + 1)) / W) : K;
+    const int i = min(g0 + lane, K - 1);
+    const CDoubles d = as_const(ad + s);
+    const CPairs rst = as_const(rs + s);
+    const double di = ad[s + i];
     double prod = 1.0;
-    for (int j = c; j < K; j += L) {
-        double dl = (di - rdorg[s + j]) - rtau[s + j];
-        const int a = (j == K - 1) ? K - 2 : j;
-        if ((unsigned)(i - a) <= 1u) dl = (i == a) ? rn1[s + j] : rn2[s + j];
-        prod *= (j == i) ? dl : dl * rcp_nr(di - d[j]);
+    // roots whose near1/near2 apply to a pole of this group: j in [g0-1, g0+63]
+    // (near1 at pole j, near2 at pole j+1) and the last root (poles K-2, K-1)
+    const int wa = max(j0, g0 - 1), wb = min(j1, g0 + kGroup);
+    auto term = [&](double2 r, double dj, int j, bool check) {
+        double dl = (di - r.x) - r.y;
+        if (check) {
+            if (i == j) dl = rn1[s + j];
+            else if (i == j + 1) dl = rn2[s + j];
+        }
+        prod *= (j == i) ? dl : dl * rcp_nr(di - dj);
+    };
+    auto range = [&](int ja, int jb, bool check) {
+        int j = ja;
+        for (; j + kBatch <= jb; j += kBatch) {
+            double2 r[kBatch];
+            double dj[kBatch];
+#pragma unroll
+            for (int b = 0; b < kBatch; b++) {
+                r[b] = rst[j + b];
+                dj[b] = d[j + b];
+            }
+#pragma unroll
+            for (int b = 0; b < kBatch; b++) term(r[b], dj[b], j + b, check);
+        }
+        for (; j < jb; j++) term(rst[j], d[j], j, check);
+    };
+    const int jlast = (K - 1 >= j0 && K - 1 < j1) ? K - 1 : -1;
+    const int jend = (jlast >= 0) ? K - 1 : j1;  // the last root is handled apart
+    if (wa < wb) {
+        range(j0, min(wa, jend), false);
+        range(wa, min(wb, jend), true);
+        range(max(wb, j0), jend, false);
+    } else {
+        range(j0, jend, false);
     }
-    for (int o = 1; o < L; o <<= 1) prod *= __shfl_xor(prod, o, kWave);
-    if (c == 0) w[s + i] = copysign(sqrt(-prod), az[s + i]);
+    if (jlast >= 0) {  // root K-1: near1 at pole K-2, near2 at pole K-1
+        const double2 r = rs[s + jlast];
+        double dl = (di - r.x) - r.y;
+        if (i == K - 2) dl = rn1[s + jlast];
+        else if (i == K - 1) dl = rn2[s + jlast];
+        prod *= (jlast == i) ? dl : dl * rcp_nr(di - ad[s + jlast]);
+    }
+    if (W > 1) {
+        red[c][lane] = prod;
+        __syncthreads();
+        prod = 1.0;
+        for (int cc = 0; cc < W; cc++) prod *= red[cc][lane];
+    }
+    if (c == 0 && g0 + lane < K) pw[s + i] = make_double2(di, copysign(sqrt(-prod), az[s + i]));
 }
 
 // Merge step 5: the two boundary rows of the merged eigenvector block
-// (dlaed9_br.f:272-331): L lanes per root j streaming the poles i,
+// (dlaed9_br.f:272-331): one lane per root j streaming the poles i,
 // x_i = w_i / delta_j(i), row_r(j) = sum_i q_r(i) x_i / ||x||.
-__global__ __launch_bounds__(kThreads) void k_rows(
-    int n, int depth, int bpm, const int* __restrict__ counts, const double* __restrict__ ad,
-    const double2* __restrict__ aq, const double* __restrict__ w,
-    const double* __restrict__ rtau, const double* __restrict__ rn1,
-    const double* __restrict__ rn2, const double* __restrict__ rdorg,
-    double* __restrict__ qo1, double* __restrict__ qo2) {
-    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    const int K = counts[4 * merge];
-    const int L = lanes_per_item(K, bpm * kThreads);
-    const int per_block = kThreads / L;
-    if (part * per_block >= K) return;
+template <int WL>
+__global__ __launch_bounds__(kGroup * WL) void k_rows(
+    int n, int depth, int bpg, const int* __restrict__ counts, const double2* __restrict__ aq,
+    const double2* __restrict__ pw, const double2* __restrict__ rs,
+    const double* __restrict__ rn1, const double* __restrict__ rn2, double* __restrict__ qo1,
+    double* __restrict__ qo2) {
+    __shared__ double red[WL > 1 ? WL : 1][3][kGroup];
+    const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
+    const int K = uniform(counts[4 * merge]);
+    const int g0 = g * kGroup;
+    if (g0 >= K) return;
     int s, sz;
     node_geom(n, depth, merge, &s, &sz);
-    const int c = threadIdx.x & (L - 1);
-    const int j = part * per_block + threadIdx.x / L;
-    if (j >= K) return;
-    if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172); L == 1
+    const int lane = threadIdx.x & (kGroup - 1);
+    const int c = uniform(threadIdx.x / kGroup);
+    if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172)
+        const int j = g0 + lane;
+        if (c != 0 || j >= K) return;
         const double v0 = rn1[s + j], v1 = rn2[s + j];
         const double2 q0 = aq[s];
         double a1 = q0.x * v0, a2 = q0.y * v0;
@@ -787,27 +974,67 @@ __global__ __launch_bounds__(kThreads) void k_rows(
         qo2[s + j] = a2;
         return;
     }
-    const double tau = rtau[s + j], nr1 = rn1[s + j], nr2 = rn2[s + j];
-    const double dorg = rdorg[s + j];
+    const int W = waves_for(K, WL);
+    if (W == 1 && c > 0) return;
+    const int i0 = (c < W) ? (int)(((long long)K * c) / W) : K;
+    const int i1 = (c < W) ? (int)(((long long)K * (c + 1)) / W) : K;
+    const int j = min(g0 + lane, K - 1);
+    const double2 r = rs[s + j];
+    const double dorg = r.x, tau = r.y, nr1 = rn1[s + j], nr2 = rn2[s + j];
     const int a = (j == K - 1) ? K - 2 : j;
-    const double* __restrict__ d = ad + s;
-    const double* __restrict__ wv = w + s;
-    const double2* __restrict__ q = aq + s;
+    const CPairs P = as_const(pw + s);
+    const CPairs Q = as_const(aq + s);
     double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
-#pragma unroll 2
-    for (int i = c; i < K; i += L) {
-        double dl = (d[i] - dorg) - tau;
-        if ((unsigned)(i - a) <= 1u) dl = (i == a) ? nr1 : nr2;
-        const double x = wv[i] * rcp_nr(dl);
-        const double2 qi = q[i];
+    auto term = [&](double2 p, double2 q, int i, bool check) {
+        double dl = (p.x - dorg) - tau;
+        if (check) {
+            if (i == a) dl = nr1;
+            else if (i == a + 1) dl = nr2;
+        }
+        const double x = p.y * rcp_nr(dl);
         temp = fma(x, x, temp);
-        acc1 = fma(qi.x, x, acc1);
-        acc2 = fma(qi.y, x, acc2);
+        acc1 = fma(q.x, x, acc1);
+        acc2 = fma(q.y, x, acc2);
+    };
+    auto range = [&](int ia, int ib, bool check) {
+        int i = ia;
+        for (; i + kBatch <= ib; i += kBatch) {
+            double2 p[kBatch], q[kBatch];
+#pragma unroll
+            for (int b = 0; b < kBatch; b++) {
+                p[b] = P[i + b];
+                q[b] = Q[i + b];
+            }
+#pragma unroll
+            for (int b = 0; b < kBatch; b++) term(p[b], q[b], i + b, check);
+        }
+        for (; i < ib; i++) term(P[i], Q[i], i, check);
+    };
+    // poles that may be pole-adjacent to a root of this group: [g0, g0+64]
+    // and the last root's K-2, K-1
+    const int wa = max(i0, g0), wb = min(i1, g0 + kGroup + 1);
+    const int tail = max(i0, min(i1, K - 2));  // from here on always checked
+    if (wa < wb) {
+        range(i0, min(wa, tail), false);
+        range(wa, min(wb, tail), true);
+        range(max(wb, i0), tail, false);
+    } else {
+        range(i0, tail, false);
     }
-    temp = group_sum(temp, L);
-    acc1 = group_sum(acc1, L);
-    acc2 = group_sum(acc2, L);
-    if (c == 0) {
+    range(tail, i1, true);
+    if (W > 1) {
+        red[c][0][lane] = temp;
+        red[c][1][lane] = acc1;
+        red[c][2][lane] = acc2;
+        __syncthreads();
+        temp = acc1 = acc2 = 0.0;
+        for (int cc = 0; cc < W; cc++) {
+            temp += red[cc][0][lane];
+            acc1 += red[cc][1][lane];
+            acc2 += red[cc][2][lane];
+        }
+    }
+    if (c == 0 && g0 + lane < K) {
         const double nrm = sqrt(temp);
         qo1[s + j] = acc1 / nrm;
         qo2[s + j] = acc2 / nrm;

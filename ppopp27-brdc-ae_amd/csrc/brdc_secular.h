@@ -86,8 +86,12 @@ __host__ __device__ inline void secular_two(int i, const double* d, const double
 
 // Three-pole step (the DLAED6 role): root of c + sum_k zz_k / (dd_k - t)
 // between dd[1], dd[2] (orgati) or dd[0], dd[1], starting from f(0) = finit.
-__host__ __device__ inline int secular_three(int kniter, bool orgati, double rho, const double dd[3],
-                                    const double zz[3], double finit, double* tau_out) {
+__host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, double rho,
+                                                     double dd0, double dd1, double dd2,
+                                                     double zz0, double zz1, double zz2,
+                                                     double finit, double* tau_out) {
+    const double dd[3] = {dd0, dd1, dd2};
+    const double zz[3] = {zz0, zz1, zz2};
     double lbd, ubd, tau = 0.0, a, b, c, eta, f, df, ddf, fc, erretm, temp;
     double ds[3], zs[3];
     int info = 0;
@@ -154,6 +158,7 @@ __host__ __device__ inline int secular_three(int kniter, bool orgati, double rho
             sclfac = 0x1p340;
             sclinv = 0x1p-340;
         }
+#pragma unroll
         for (int k = 0; k < 3; k++) {
             ds[k] = dd[k] * sclfac;
             zs[k] = zz[k] * sclfac;
@@ -162,12 +167,14 @@ __host__ __device__ inline int secular_three(int kniter, bool orgati, double rho
         lbd *= sclfac;
         ubd *= sclfac;
     } else {
+#pragma unroll
         for (int k = 0; k < 3; k++) {
             ds[k] = dd[k];
             zs[k] = zz[k];
         }
     }
     fc = df = ddf = 0.0;
+#pragma unroll
     for (int k = 0; k < 3; k++) {
         const double r = 1.0 / (ds[k] - tau);
         const double t1 = zs[k] * r, t2 = t1 * r, t3 = t2 * r;
@@ -210,9 +217,15 @@ __host__ __device__ inline int secular_three(int kniter, bool orgati, double rho
         if (tau < lbd || tau > ubd)
             tau = (lbd + ubd) / 2.0;
         fc = erretm = df = ddf = 0.0;
+        bool hit = false;
+#pragma unroll
         for (int k = 0; k < 3; k++) {
-            if (ds[k] - tau == 0.0)
-                goto done;
+            if (ds[k] - tau == 0.0) hit = true;
+        }
+        if (hit)
+            goto done;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
             const double r = 1.0 / (ds[k] - tau);
             const double u1 = zs[k] * r, u2 = u1 * r, u3 = u2 * r, u4 = u1 / ds[k];
             fc += u4;
@@ -264,193 +277,152 @@ struct SecSums {
     double S, A, dpsi, dphi, J;
 };
 
-// Root i (0-based) of the K-pole secular equation, K >= 3.  E provides
-// d(j), zsq(j) = z_j^2 and sums(d_org, tau, ex_lo, nex).
+// Root i (0-based) of the K-pole secular equation, K >= 3, as a state
+// machine around a single pole-pass call site: every unconverged root of a
+// wavefront streams the poles in the same pass, whatever phase it is in.
+// E provides d(j), zsq(j) = z_j^2 and sums(d_org, tau, ex_lo, nex).
+//
+// Pass sequence (LAPACK DLAED4): E0 at the interval midpoint gives the
+// initial guess; E1 evaluates it; after each E_k (k >= 1) the next step
+// number niter = k+1 is tested (|w| <= eps*erretm), failing past 30 steps,
+// then the step is taken and E_{k+1} evaluates the new iterate.
 template <class Eval>
-__host__ __device__ RootState secular_root(const Eval& E, int K, int i, double rho) {
-    RootState r;
-    r.info = 0;
-    r.iters = 0;
-    const double rhoinv = 1.0 / rho;
-    if (i == K - 1) {
-        // ---- last root, in (d_{K-1}, d_{K-1} + rho)
-        const double dk = E.d(K - 1), dk1 = E.d(K - 2);
-        const double zk2 = E.zsq(K - 1), zk12 = E.zsq(K - 2);
-        const double midpt = rho / 2.0;
-        double tau, dltlb, dltub;
-        {
-            const SecSums s0 = E.sums(dk, midpt, K - 2, 2);
-            r.iters++;
-            const double c = rhoinv + s0.S;
-            const double dl_ii = (dk1 - dk) - midpt, dl_n = -midpt;
-            const double w = c + zk12 / dl_ii + zk2 / dl_n;
-            const double del = dk - dk1;
-            const double a = -c * del + zk12 + zk2;
-            const double b = zk2 * del;
-            if (w <= 0.0) {
-                const double temp = zk12 / (dk - dk1 + rho) + zk2 / rho;
-                if (c <= temp)
-                    tau = rho;
-                else if (a < 0.0)
-                    tau = 2.0 * b / (sqrt(a * a + 4.0 * b * c) - a);
-                else
-                    tau = (a + sqrt(a * a + 4.0 * b * c)) / (2.0 * c);
-                dltlb = midpt;
-                dltub = rho;
-            } else {
-                if (a < 0.0)
-                    tau = 2.0 * b / (sqrt(a * a + 4.0 * b * c) - a);
-                else
-                    tau = (a + sqrt(a * a + 4.0 * b * c)) / (2.0 * c);
-                dltlb = 0.0;
-                dltub = midpt;
-            }
+struct SecularSolve {
+    // problem
+    int K, i;
+    double rho, rhoinv;
+    bool last;
+    // interval / origin
+    double di, dip1, zi2, zip12, del, midpt;  // interior: d_i, d_{i+1}; last: d_{K-1}, d_{K-2}
+    bool orgati;
+    int ii;
+    double dorg, zii2, dm1, zm12, dp1, zp12;
+    // iterate
+    double tau, dltlb, dltub;
+    double S, A, dpsi, dphi, erretm, w, w0, dw, psi, phi, prew;
+    bool swtch3, swtch;
+    int k;  // passes done
+    // next pass
+    double ev_org, ev_tau;
+    int ev_exlo, ev_nex;
+    RootState out;
+
+    __host__ __device__ void init(const Eval& E, int K_, int i_, double rho_) {
+        K = K_;
+        i = i_;
+        rho = rho_;
+        rhoinv = 1.0 / rho;
+        last = (i == K - 1);
+        out.info = 0;
+        out.iters = 0;
+        k = 0;
+        swtch3 = swtch = false;
+        if (last) {
+            di = E.d(K - 1);   // d_{K-1}
+            dip1 = E.d(K - 2); // d_{K-2}
+            zi2 = E.zsq(K - 1);
+            zip12 = E.zsq(K - 2);
+            midpt = rho / 2.0;
+            ev_org = di;
+            ev_tau = midpt;
+            ev_exlo = K - 2;
+            ev_nex = 2;
+        } else {
+            di = E.d(i);
+            dip1 = E.d(i + 1);
+            zi2 = E.zsq(i);
+            zip12 = E.zsq(i + 1);
+            del = dip1 - di;
+            midpt = del / 2.0;
+            ev_org = di;
+            ev_tau = midpt;
+            ev_exlo = i;
+            ev_nex = 2;
         }
-        double psi, dpsi, phi, dphi, erretm, w;
-        auto eval = [&]() {
-            const SecSums s = E.sums(dk, tau, K - 1, 1);
-            r.iters++;
+    }
+
+    __host__ __device__ double delta_at(double dj) const { return (dj - dorg) - tau; }
+
+    __host__ __device__ void finish(bool converged) {
+        if (!converged && out.info == 0) out.info = 1;
+        out.lambda = dorg + tau;
+        out.org = ii;
+        out.tau = tau;
+        if (last) {
+            out.near1 = (dip1 - dorg) - tau;
+            out.near2 = -tau;
+        } else {
+            out.near1 = (di - dorg) - tau;
+            out.near2 = (dip1 - dorg) - tau;
+        }
+    }
+
+    // Consume pass k's sums; returns true when the root is finished.
+    __host__ __device__ bool advance(const Eval& E, const SecSums& s) {
+        out.iters++;
+        if (k == 0) {
+            k = 1;
+            return initial_guess(E, s);
+        }
+        if (last) {
             psi = s.S;
             dpsi = s.dpsi + s.dphi;
             const double err = s.J - (double)(K - 1) * s.S;
             const double dn = -tau;
-            phi = zk2 / dn;
+            phi = zi2 / dn;
             dphi = phi / dn;
             erretm = 8.0 * (-phi - psi) + err - phi + rhoinv + fabs(tau) * (dpsi + dphi);
             w = rhoinv + phi + psi;
-        };
-        eval();
-        bool conv = false;
-        for (int niter = 2; niter <= kSecularMaxIt; niter++) {
-            if (fabs(w) <= kEps * erretm) {
-                conv = true;
-                break;
+        } else {
+            S = s.S;
+            A = s.A;
+            dpsi = s.dpsi;
+            dphi = s.dphi;
+            const double err = s.J - (double)ii * s.S;
+            const double dii = -tau;
+            const double temp = zii2 / dii;
+            dw = dpsi + dphi + temp / dii;
+            w0 = rhoinv + S;
+            w = w0 + temp;
+            erretm = 8.0 * A + err + 2.0 * rhoinv + 3.0 * fabs(temp) + (k >= 2 ? fabs(tau) : 0.0) * dw;
+            if (k == 1) {
+                swtch3 = orgati ? (w0 < 0.0) : (w0 > 0.0);
+                if (ii == 0 || ii == K - 1) swtch3 = false;
+            } else if (k == 2) {
+                swtch = orgati ? (-w > fabs(prew) / 10.0) : (w > fabs(prew) / 10.0);
+            } else if (w * prew > 0.0 && fabs(w) > fabs(prew) / 10.0) {
+                swtch = !swtch;
             }
-            if (w <= 0.0)
-                dltlb = fmax(dltlb, tau);
-            else
-                dltub = fmin(dltub, tau);
-            const double dn1 = (dk1 - dk) - tau, dn = -tau;
+        }
+        const int niter = k + 1;
+        if (niter > kSecularMaxIt) {
+            finish(false);
+            return true;
+        }
+        if (fabs(w) <= kEps * erretm) {
+            finish(true);
+            return true;
+        }
+        if (w <= 0.0)
+            dltlb = fmax(dltlb, tau);
+        else
+            dltub = fmin(dltub, tau);
+        double eta;
+        if (last) {
+            const double dn1 = (dip1 - di) - tau, dn = -tau;
             double c = w - dn1 * dpsi - dn * dphi;
             const double a = (dn1 + dn) * w - dn1 * dn * (dpsi + dphi);
             const double b = dn1 * dn * w;
-            if (c < 0.0)
-                c = fabs(c);
-            double eta;
+            if (c < 0.0) c = fabs(c);
             if (c == 0.0)
                 eta = -w / (dpsi + dphi);
             else if (a >= 0.0)
                 eta = (a + sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
             else
                 eta = 2.0 * b / (a - sqrt(fabs(a * a - 4.0 * b * c)));
-            if (w * eta > 0.0)
-                eta = -w / (dpsi + dphi);
-            const double temp = tau + eta;
-            if (temp > dltub || temp < dltlb)
-                eta = (w < 0.0) ? (dltub - tau) / 2.0 : (dltlb - tau) / 2.0;
-            tau += eta;
-            eval();
-        }
-        if (!conv)
-            r.info = 1;
-        r.lambda = dk + tau;
-        r.org = K - 1;
-        r.tau = tau;
-        r.near1 = (dk1 - dk) - tau;
-        r.near2 = -tau;
-        return r;
-    }
-
-    // ---- interior root, in (d_i, d_{i+1})
-    const int ip1 = i + 1;
-    const double di = E.d(i), dip1 = E.d(ip1);
-    const double zi2 = E.zsq(i), zip12 = E.zsq(ip1);
-    const double del = dip1 - di;
-    const double midpt = del / 2.0;
-    bool orgati;
-    double tau, dltlb, dltub;
-    {
-        const SecSums s0 = E.sums(di, midpt, i, 2);
-        r.iters++;
-        const double c = rhoinv + s0.S;
-        const double w = c + zi2 / (-midpt) + zip12 / (del - midpt);
-        if (w > 0.0) {
-            orgati = true;
-            const double a = c * del + zi2 + zip12;
-            const double b = zi2 * del;
-            if (a > 0.0)
-                tau = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
-            else
-                tau = (a - sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
-            dltlb = 0.0;
-            dltub = midpt;
+            if (w * eta > 0.0) eta = -w / (dpsi + dphi);
         } else {
-            orgati = false;
-            const double a = c * del - zi2 - zip12;
-            const double b = zip12 * del;
-            if (a < 0.0)
-                tau = 2.0 * b / (a - sqrt(fabs(a * a + 4.0 * b * c)));
-            else
-                tau = -(a + sqrt(fabs(a * a + 4.0 * b * c))) / (2.0 * c);
-            dltlb = -midpt;
-            dltub = 0.0;
-        }
-    }
-    const int ii = orgati ? i : ip1;
-    const int iim1 = ii - 1, iip1 = ii + 1;
-    const double dorg = orgati ? di : dip1;
-    const double zii2 = E.zsq(ii);
-    // neighbour poles of the three-pole model (only used when 0 < ii < K-1)
-    const double dm1 = (ii > 0) ? E.d(iim1) : 0.0;
-    const double zm12 = (ii > 0) ? E.zsq(iim1) : 0.0;
-    const double dp1 = (iip1 < K) ? E.d(iip1) : 0.0;
-    const double zp12 = (iip1 < K) ? E.zsq(iip1) : 0.0;
-
-    double A, S, dpsi, dphi, erretm, w, w0, dw;
-    // w0: secular function without the origin pole's term; w: with it
-    auto eval = [&](double tau_err) {
-        const SecSums s = E.sums(dorg, tau, ii, 1);
-        r.iters++;
-        S = s.S;
-        A = s.A;
-        dpsi = s.dpsi;
-        dphi = s.dphi;
-        const double err = s.J - (double)ii * s.S;
-        const double dii = -tau;  // delta at the origin pole
-        const double temp = zii2 / dii;
-        dw = dpsi + dphi + temp / dii;
-        w0 = rhoinv + S;
-        w = w0 + temp;
-        erretm = 8.0 * A + err + 2.0 * rhoinv + 3.0 * fabs(temp) + tau_err * dw;
-    };
-    eval(0.0);
-    bool swtch3 = orgati ? (w0 < 0.0) : (w0 > 0.0);
-    if (ii == 0 || ii == K - 1)
-        swtch3 = false;
-
-    auto delta_at = [&](double dj) { return (dj - dorg) - tau; };
-    bool conv = false;
-    if (fabs(w) <= kEps * erretm) {
-        conv = true;
-    } else {
-        if (w <= 0.0)
-            dltlb = fmax(dltlb, tau);
-        else
-            dltub = fmin(dltub, tau);
-        bool swtch = false;
-        for (int niter = 2; niter <= kSecularMaxIt; niter++) {
-            if (niter > 2) {
-                if (fabs(w) <= kEps * erretm) {
-                    conv = true;
-                    break;
-                }
-                if (w <= 0.0)
-                    dltlb = fmax(dltlb, tau);
-                else
-                    dltub = fmin(dltub, tau);
-            }
             const double dl_i = delta_at(di), dl_ip1 = delta_at(dip1);
-            double eta;
             if (!swtch3) {
                 double c;
                 if (niter == 2 || !swtch) {
@@ -489,52 +461,122 @@ __host__ __device__ RootState secular_root(const Eval& E, int K, int i, double r
             } else {
                 const double dl_m = delta_at(dm1), dl_p = delta_at(dp1);
                 const double temp = rhoinv + S;
-                double c, zz[3];
+                double c, zz0, zz2;
                 if (niter > 2 && swtch) {
                     c = temp - dl_m * dpsi - dl_p * dphi;
-                    zz[0] = dl_m * dl_m * dpsi;
-                    zz[2] = dl_p * dl_p * dphi;
+                    zz0 = dl_m * dl_m * dpsi;
+                    zz2 = dl_p * dl_p * dphi;
                 } else if (orgati) {
                     const double t1 = zm12 / (dl_m * dl_m);
                     c = temp - dl_p * (dpsi + dphi) - (dm1 - dp1) * t1;
-                    zz[0] = zm12;
-                    zz[2] = dl_p * dl_p * ((dpsi - t1) + dphi);
+                    zz0 = zm12;
+                    zz2 = dl_p * dl_p * ((dpsi - t1) + dphi);
                 } else {
                     const double t1 = zp12 / (dl_p * dl_p);
                     c = temp - dl_m * (dpsi + dphi) - (dp1 - dm1) * t1;
-                    zz[0] = dl_m * dl_m * (dpsi + (dphi - t1));
-                    zz[2] = zp12;
+                    zz0 = dl_m * dl_m * (dpsi + (dphi - t1));
+                    zz2 = zp12;
                 }
-                zz[1] = zii2;
-                const double dd3[3] = {dl_m, delta_at(dorg), dl_p};
-                if (secular_three(niter, orgati, c, dd3, zz, w, &eta)) {
-                    r.info = 1;
-                    break;
+                if (secular_three(niter, orgati, c, dl_m, delta_at(dorg), dl_p, zz0, zii2, zz2, w,
+                                  &eta)) {
+                    out.info = 1;
+                    finish(false);
+                    return true;
                 }
             }
-            if (w * eta >= 0.0)
-                eta = -w / dw;
-            const double temp = tau + eta;
-            if (temp > dltub || temp < dltlb)
-                eta = (w < 0.0) ? (dltub - tau) / 2.0 : (dltlb - tau) / 2.0;
-            const double prew = w;
-            tau += eta;
-            eval(fabs(tau));
-            if (niter == 2) {
-                swtch = orgati ? (-w > fabs(prew) / 10.0) : (w > fabs(prew) / 10.0);
-            } else if (w * prew > 0.0 && fabs(w) > fabs(prew) / 10.0) {
-                swtch = !swtch;
-            }
+            if (w * eta >= 0.0) eta = -w / dw;
         }
+        const double temp = tau + eta;
+        if (temp > dltub || temp < dltlb)
+            eta = (w < 0.0) ? (dltub - tau) / 2.0 : (dltlb - tau) / 2.0;
+        prew = w;
+        tau += eta;
+        k++;
+        ev_tau = tau;
+        return false;
     }
-    if (!conv && r.info == 0)
-        r.info = 1;
-    r.lambda = dorg + tau;
-    r.org = ii;
-    r.tau = tau;
-    r.near1 = (di - dorg) - tau;
-    r.near2 = (dip1 - dorg) - tau;
-    return r;
+
+    __host__ __device__ bool initial_guess(const Eval& E, const SecSums& s0) {
+        if (last) {
+            const double c = rhoinv + s0.S;
+            const double dl_ii = (dip1 - di) - midpt, dl_n = -midpt;
+            const double w_ = c + zip12 / dl_ii + zi2 / dl_n;
+            const double dl = di - dip1;
+            const double a = -c * dl + zip12 + zi2;
+            const double b = zi2 * dl;
+            if (w_ <= 0.0) {
+                const double temp = zip12 / (di - dip1 + rho) + zi2 / rho;
+                if (c <= temp)
+                    tau = rho;
+                else if (a < 0.0)
+                    tau = 2.0 * b / (sqrt(a * a + 4.0 * b * c) - a);
+                else
+                    tau = (a + sqrt(a * a + 4.0 * b * c)) / (2.0 * c);
+                dltlb = midpt;
+                dltub = rho;
+            } else {
+                if (a < 0.0)
+                    tau = 2.0 * b / (sqrt(a * a + 4.0 * b * c) - a);
+                else
+                    tau = (a + sqrt(a * a + 4.0 * b * c)) / (2.0 * c);
+                dltlb = 0.0;
+                dltub = midpt;
+            }
+            ii = K - 1;
+            dorg = di;
+            ev_org = dorg;
+            ev_tau = tau;
+            ev_exlo = K - 1;
+            ev_nex = 1;
+            return false;
+        }
+        const double c = rhoinv + s0.S;
+        const double w_ = c + zi2 / (-midpt) + zip12 / (del - midpt);
+        if (w_ > 0.0) {
+            orgati = true;
+            const double a = c * del + zi2 + zip12;
+            const double b = zi2 * del;
+            if (a > 0.0)
+                tau = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
+            else
+                tau = (a - sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
+            dltlb = 0.0;
+            dltub = midpt;
+        } else {
+            orgati = false;
+            const double a = c * del - zi2 - zip12;
+            const double b = zip12 * del;
+            if (a < 0.0)
+                tau = 2.0 * b / (a - sqrt(fabs(a * a + 4.0 * b * c)));
+            else
+                tau = -(a + sqrt(fabs(a * a + 4.0 * b * c))) / (2.0 * c);
+            dltlb = -midpt;
+            dltub = 0.0;
+        }
+        ii = orgati ? i : i + 1;
+        dorg = orgati ? di : dip1;
+        zii2 = orgati ? zi2 : zip12;
+        dm1 = (ii > 0) ? E.d(ii - 1) : 0.0;
+        zm12 = (ii > 0) ? E.zsq(ii - 1) : 0.0;
+        dp1 = (ii + 1 < K) ? E.d(ii + 1) : 0.0;
+        zp12 = (ii + 1 < K) ? E.zsq(ii + 1) : 0.0;
+        ev_org = dorg;
+        ev_tau = tau;
+        ev_exlo = ii;
+        ev_nex = 1;
+        return false;
+    }
+};
+
+template <class Eval>
+__host__ __device__ RootState secular_root(const Eval& E, int K, int i, double rho) {
+    SecularSolve<Eval> st;
+    st.init(E, K, i, rho);
+    for (;;) {
+        const SecSums s = E.sums(st.ev_org, st.ev_tau, st.ev_exlo, st.ev_nex);
+        if (st.advance(E, s)) break;
+    }
+    return st.out;
 }
 
 }  // namespace brdc
