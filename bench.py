@@ -252,7 +252,7 @@ def main():
         "data": "synthetic (splitmix64 generators, seed 0xB7DC0000+2000+N)",
         "config": {"workload": f"plain BR values-only, N={n}, one solve per generator "
                                f"({', '.join(GENS)}) per step",
-                   "n": n, "generators": list(GENS), "leaf_max": solver.opts.leaf_max or 32,
+                   "n": n, "generators": list(GENS), "leaf_max": solver.opts.leaf_max or 16,
                    "launch": "eager" if args.eager else "hipGraph replay",
                    "per_generator_ms": {g: round(v, 4) for g, v in per_gen.items()},
                    "per_generator_ms_eager_launch": {g: round(v, 4)

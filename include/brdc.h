@@ -33,7 +33,7 @@ typedef enum brdc_status {
 /* Tuning knobs of the device path; zero-initialise for the defaults. */
 typedef struct brdc_options {
     int leaf_max;     /* largest leaf (rows) solved by the wavefront Jacobi
-                         kernel, 2..64; 0 selects the default (32) */
+                         kernel, 2..64; 0 selects the default (16) */
     int reserved[7];  /* must be zero */
 } brdc_options;
 

@@ -176,3 +176,28 @@ def lamrg(a, n1: int, n2: int, s1: int = 1, s2: int = 1):
     idx = np.zeros(n1 + n2, dtype=np.int32)
     lib().brdc_oracle_lamrg(n1, n2, _p(a), s1, s2, _p(idx))
     return idx
+
+
+MERGE_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                            _dp, _dp)
+
+
+def merge_problems(d, e, smlsiz: int = 0):
+    """Every merge's secular problem of the BR restatement on (d, e):
+    a list of (level, start, rho, dlambda[K], w[K]) in the order solved."""
+    out = []
+
+    def cb(level, start, k, rho, dl, w):
+        out.append((level, start, rho, np.ctypeslib.as_array(dl, shape=(k,)).copy(),
+                    np.ctypeslib.as_array(w, shape=(k,)).copy()))
+
+    f = MERGE_CB(cb)
+    L = lib()
+    L.brdc_oracle_set_merge_callback.argtypes = [MERGE_CB]
+    L.brdc_oracle_set_merge_callback.restype = None
+    L.brdc_oracle_set_merge_callback(f)
+    try:
+        dstebr(d, e, smlsiz=smlsiz, nthreads=1)
+    finally:
+        L.brdc_oracle_set_merge_callback(MERGE_CB())
+    return out

@@ -1250,6 +1250,10 @@ static int laed9_br(int k, int nbr, double *d, const double *qin, double *qout,
     return 0;
 }
 
+static brdc_oracle_merge_cb g_merge_cb = NULL;
+static int g_cb_level = 0, g_cb_start = 0;
+void brdc_oracle_set_merge_callback(brdc_oracle_merge_cb cb) { g_merge_cb = cb; }
+
 /* DLAED7_BR (dlaed7_br.f:7-167): one merge of two sorted children. */
 static int laed7_br(int wantq, int n, int cut, double *d, int *indxq,
                     double rho, double *blo, double *bhi, merge_ws *ws,
@@ -1281,6 +1285,8 @@ static int laed7_br(int wantq, int n, int cut, double *d, int *indxq,
     if (info)
         return info;
     *kdefl = k;
+    if (k != 0 && g_merge_cb)
+        g_merge_cb(g_cb_level, g_cb_start, k, rho, ws->dlambda, ws->w);
     if (k != 0) {
         info = laed9_br(k, nbr, d, ws->q2, q, rho, ws->dlambda, ws->w, ws,
                         nthreads);
@@ -1382,6 +1388,8 @@ static int laed0_br(int n, double *d, double *e, int sml, int nthreads,
             if (merge_ws_alloc(&ws, msz, (outer > 1) ? 1 : wthr) != 0) {
                 infom = -99;
             } else {
+                g_cb_level = level;
+                g_cb_start = sub;
                 infom = laed7_br(wantq, msz, cut, d + sub, indxq + sub,
                                  e[sub + cut - 1], blo + sub, bhi + sub, &ws,
                                  inner, &kd, &ng);

@@ -47,6 +47,14 @@ int brdc_oracle_dstebr_ex(int n, double *d, double *e, double *work, int lwork,
 int brdc_oracle_dstebr(int n, double *d, double *e, double *work, int lwork,
                        int *iwork, int liwork);
 
+/* Test hook: when set, every merge's secular problem (level, first row, K,
+ * rho, poles dlambda[K], weights w[K]) is passed to the callback just before
+ * its roots are solved.  Not thread-safe; call brdc_oracle_dstebr_ex with
+ * nthreads = 1 while it is set. */
+typedef void (*brdc_oracle_merge_cb)(int level, int start, int k, double rho,
+                                     const double *dlambda, const double *w);
+void brdc_oracle_set_merge_callback(brdc_oracle_merge_cb cb);
+
 /* Secular equation roots (LAPACK DLAED4/DLAED5/DLAED6 restated).  i is
  * 0-based.  Return value is LAPACK's INFO. */
 int brdc_oracle_laed4(int n, int i, const double *d, const double *z,

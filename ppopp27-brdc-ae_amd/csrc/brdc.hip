@@ -21,8 +21,9 @@
 namespace brdc {
 namespace {
 
-constexpr int kDefaultLeaf = 32;
-constexpr int kSecWaves = 4;  // waves per block of the secular kernel (independent waves)
+constexpr int kDefaultLeaf = 16;
+constexpr int kSecWaves = 4;       // waves per block of the secular kernel (independent waves)
+constexpr int kSecMaxWaves = 3072;  // secular grid cap: about 3 resident waves per SIMD
 constexpr size_t kAlign = 256;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
@@ -253,10 +254,13 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         BRDC_LAUNCH_CHECK();
         {
             ProfScope ps(kSecular, st);
-            // waves needed per merge: K * L / 64 <= maxsz roots' worth at L = 64
-            const int bpm_sec = (maxsz + kSecWaves - 1) / kSecWaves;
-            hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(merges * bpm_sec), dim3(kWave * kSecWaves),
-                               0, st, n, depth, bpm_sec, (const double*)ew, (const int*)counts,
+            // wave slots per merge: the most any K <= maxsz can need; the grid
+            // is capped and walks (merge, slot) tasks
+            const int slots = secular_slots(maxsz, merges);
+            const int waves = std::min(merges * slots, kSecMaxWaves);
+            const int blocks = (waves + kSecWaves - 1) / kSecWaves;
+            hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves), 0, st,
+                               n, depth, slots, (const double*)ew, (const int*)counts,
                                (const double*)ad, (const double*)az, (const double2*)adz, rlam,
                                rs, rn1, rn2, counts, info);
         }

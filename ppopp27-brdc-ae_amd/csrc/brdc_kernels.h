@@ -19,12 +19,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include "brdc_launch.h"
 #include "brdc_secular.h"
 #include "brdc_tree.h"
 
 namespace brdc {
 
-constexpr int kWave = 64;
 constexpr int kMaxLeaf = 64;
 constexpr int kJacobiSweeps = 40;
 constexpr int kSpecCap = 256;     // speculative Givens-run walk bound
@@ -203,28 +203,32 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
     const int mm = m + (m & 1);
     const int npair = mm / 2;
     const int nblk = npair * npair;
+    const float inv_npair = 1.0f / (float)npair;
     bool converged = (m <= 1);
     for (int sweep = 0; sweep < kJacobiSweeps && !converged; sweep++) {
         if (lane == 0) changed = 0;
         __syncthreads();
         for (int r = 0; r < mm - 1; r++) {
             if (lane < npair) {
-                const int a = (lane == 0) ? 0 : ((lane - 1 + r) % (mm - 1)) + 1;
-                const int bs = mm - 1 - lane;
-                const int b = ((bs - 1 + r) % (mm - 1)) + 1;
+                // round-robin slots: slot 0 fixed, slots 1..mm-1 rotate by r
+                int xa = lane - 1 + r, xb = mm - 2 - lane + r;
+                if (xa >= mm - 1) xa -= mm - 1;
+                if (xb >= mm - 1) xb -= mm - 1;
+                const int a = (lane == 0) ? 0 : xa + 1;
+                const int b = xb + 1;
                 const int p = min(a, b), q = max(a, b);
                 double c = 1.0, sn = 0.0;
                 if (q < m) {
                     const double apq = A[p * LD + q];
                     const double app = A[p * LD + p], aqq = A[q * LD + q];
                     if (apq != 0.0 && fabs(apq) > kEps * sqrt(fabs(app)) * sqrt(fabs(aqq))) {
-                        const double theta = (aqq - app) / (2.0 * apq);
+                        const double theta = (aqq - app) * rcp_nr(2.0 * apq);
                         double t;
                         if (fabs(theta) > 1e150)
-                            t = 0.5 / theta;
+                            t = 0.5 * rcp_nr(theta);
                         else
-                            t = copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(t * t + 1.0);
+                            t = copysign(rcp_nr(fabs(theta) + sqrt(theta * theta + 1.0)), theta);
+                        c = rcp_nr(sqrt(t * t + 1.0));
                         sn = t * c;
                         changed = 1;
                     }
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
             }
             __syncthreads();
             for (int blk = lane; blk < nblk; blk += kWave) {
-                const int a = blk / npair, b = blk - a * npair;
+                const int a = (int)(((float)blk + 0.5f) * inv_npair), b = blk - a * npair;
                 const double sa = RS[a], sb = RS[b];
                 if (sa == 0.0 && sb == 0.0) continue;
                 const double ca = RC[a], cb = RC[b];
@@ -757,66 +761,71 @@ struct PoleView {
 
 constexpr int kBatch = 8;
 
-__device__ __forceinline__ void secular_term(double2 p, int j, double dorg, double tau, int exlo,
-                                             int nex, double acc[5]) {
-    const double delta = (p.x - dorg) - tau;
-    const double r = rcp_nr(delta);
-    double term = p.y * r;
-    if ((unsigned)(j - exlo) < (unsigned)nex) term = 0.0;
-    acc[0] += term;
-    acc[1] += fabs(term);
-    const double x = term * fabs(r);
-    acc[2] += fmin(x, 0.0);
-    acc[3] += fmax(x, 0.0);
-    acc[4] = fma((double)j, term, acc[4]);
-}
-
-// lanes per root: about 4+ poles per lane per pass, and no more lanes than
-// the level needs to fill the machine (~kTargetLanes lanes in flight)
-constexpr int kTargetLanes = 1 << 19;
-__host__ __device__ inline int secular_lanes(int K, int merges) {
-    int L = 1;
-    while (L < kWave && 4 * 2 * L <= K && (long long)merges * K * (2 * L) <= kTargetLanes) L <<= 1;
-    return L;
-}
-
-template <int WL>
-__global__ __launch_bounds__(kWave * WL) void k_secular(
-    int n, int depth, int bpm, const double* __restrict__ ew, const int* counts,
-    const double* __restrict__ ad, const double* __restrict__ az,
-    const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
-    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
-    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    const int K = uniform(counts[4 * merge]);
-    const int L = secular_lanes(K, 1 << depth);
-    const int rpw = kWave / L;  // roots per wave
-    const int wave = part * WL + (int)(threadIdx.x / kWave);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int i = wave * rpw + lane / L;  // this lane's root
-    const int c = lane & (L - 1);
-    if (wave * rpw >= K) return;  // whole wave idle
-    int s, sz;
-    node_geom(n, depth, merge, &s, &sz);
-    const int cut = sz / 2;
-    const double rho = fabs(2.0 * ew[s + cut - 1]);
-    if (K <= 2) {
-        if (i >= K || c != 0) return;
-        if (K == 1) {
-            const double z0 = az[s];
-            rlam[s] = ad[s] + rho * z0 * z0;
-            rs[s] = make_double2(ad[s], rho * z0 * z0);
-            rn1[s] = 1.0;  // secular eigenvector (1)
-            rn2[s] = 0.0;
-        } else {
-            double lam, v0, v1, tau;
-            secular_two(i, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
-            rlam[s + i] = lam;
-            rs[s + i] = make_double2(ad[s], tau);
-            rn1[s + i] = v0;  // secular eigenvector (v0, v1)
-            rn2[s + i] = v1;
+// Sums of one side of the pole list for this lane's strided poles
+// j = j_start, j_start + L, ... < j_end: S += z_j^2/delta_j, D += (z_j/delta_j)^2,
+// E += j z_j^2/delta_j.  Poles are sorted and every iterate stays inside its
+// bracket, so all poles below the origin give delta < 0 (psi side) and all
+// above give delta > 0 (phi side): the two sides are two plain loops.
+__device__ __forceinline__ void secular_side(const double2* __restrict__ dz, int j, int jend,
+                                             int L, double dorg, double tau, double& S,
+                                             double& D, double& E) {
+    double jd = (double)j;
+    const double dL = (double)L;
+    for (; j + (kBatch - 1) * L < jend; j += kBatch * L) {
+        double2 p[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) p[b] = dz[j + b * L];
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            const double r = rcp_nr((p[b].x - dorg) - tau);
+            const double term = p[b].y * r;
+            S += term;
+            D = fma(term, r, D);
+            E = fma(jd, term, E);
+            jd += dL;
         }
-        atomicAdd(&counts_rw[4 * merge + 3], 1);
-        return;
+    }
+    for (; j < jend; j += L) {
+        const double2 p = dz[j];
+        const double r = rcp_nr((p.x - dorg) - tau);
+        const double term = p.y * r;
+        S += term;
+        D = fma(term, r, D);
+        E = fma(jd, term, E);
+        jd += dL;
+    }
+}
+
+// One group of 64/L roots of one merge.  Returns the group's secular
+// iteration count (summed over the wave).
+__device__ __forceinline__ int secular_group(int n, int s, int sz, int K, int L, int grp,
+                                             double rho, const double* __restrict__ ad,
+                                             const double* __restrict__ az,
+                                             const double2* __restrict__ adz,
+                                             double* __restrict__ rlam, double2* __restrict__ rs,
+                                             double* __restrict__ rn1, double* __restrict__ rn2,
+                                             int* info) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = grp * (kWave / L) + lane / L;  // this lane's root
+    const int c = lane & (L - 1);
+    if (K <= 2) {
+        if (i < K && c == 0) {
+            if (K == 1) {
+                const double z0 = az[s];
+                rlam[s] = ad[s] + rho * z0 * z0;
+                rs[s] = make_double2(ad[s], rho * z0 * z0);
+                rn1[s] = 1.0;  // secular eigenvector (1)
+                rn2[s] = 0.0;
+            } else {
+                double lam, v0, v1, tau;
+                secular_two(i, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
+                rlam[s + i] = lam;
+                rs[s + i] = make_double2(ad[s], tau);
+                rn1[s + i] = v0;  // secular eigenvector (v0, v1)
+                rn2[s + i] = v1;
+            }
+        }
+        return (i == 0 && c == 0) ? 1 : 0;
     }
     const double2* __restrict__ dz = adz + s;
     const PoleView E{dz};
@@ -825,28 +834,22 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
     st.init(E, K, min(i, K - 1), rho);
     for (;;) {
         if (!__any(active)) break;
-        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        int j = c;
-        for (; j + (kBatch - 1) * L < K; j += kBatch * L) {
-            double2 p[kBatch];
-#pragma unroll
-            for (int b = 0; b < kBatch; b++) p[b] = dz[j + b * L];
-#pragma unroll
-            for (int b = 0; b < kBatch; b++)
-                secular_term(p[b], j + b * L, st.ev_org, st.ev_tau, st.ev_exlo, st.ev_nex, acc);
-        }
-        for (; j < K; j += L)
-            secular_term(dz[j], j, st.ev_org, st.ev_tau, st.ev_exlo, st.ev_nex, acc);
+        // psi side: poles [0, ex_lo); phi side: poles [ex_lo + nex, K)
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const int j2 = st.ev_exlo + st.ev_nex;
+        secular_side(dz, c, st.ev_exlo, L, st.ev_org, st.ev_tau, acc[0], acc[1], acc[2]);
+        secular_side(dz, j2 + ((c - j2) & (L - 1)), K, L, st.ev_org, st.ev_tau, acc[3], acc[4],
+                     acc[5]);
         for (int o = 1; o < L; o <<= 1) {
 #pragma unroll
-            for (int q = 0; q < 5; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+            for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
         }
         SecSums sums;
-        sums.S = acc[0];
-        sums.A = acc[1];
-        sums.dpsi = -acc[2];
-        sums.dphi = acc[3];
-        sums.J = acc[4];
+        sums.S = acc[0] + acc[3];
+        sums.A = acc[3] - acc[0];
+        sums.dpsi = acc[1];
+        sums.dphi = acc[4];
+        sums.J = acc[2] + acc[5];
         if (active && st.advance(E, sums)) active = false;
     }
     int iters = 0;
@@ -859,8 +862,40 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
         iters = r.iters;
         if (r.info) report_failure(info, n, s, sz);
     }
-    for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
-    if (lane == 0 && iters) atomicAdd(&counts_rw[4 * merge + 3], iters);
+    return iters;
+}
+
+// All roots of all merges of one level.  Work is split into tasks (merge,
+// slot), slot < slots; the task's wave solves root groups slot, slot + slots, ... of its
+// merge, so every group is covered whatever the slot count, and the grid is
+// capped, each wave walking tasks with a grid stride.
+template <int WL>
+__global__ __launch_bounds__(kWave * WL) void k_secular(
+    int n, int depth, int slots, const double* __restrict__ ew, const int* counts,
+    const double* __restrict__ ad, const double* __restrict__ az,
+    const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
+    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
+    const int merges = 1 << depth;
+    const int ntask = merges * slots;
+    const int nwave = gridDim.x * WL;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int task = uniform(blockIdx.x * WL + (int)(threadIdx.x / kWave)); task < ntask;
+         task += nwave) {
+        const int merge = task / slots, slot = task - merge * slots;
+        const int K = uniform(counts[4 * merge]);
+        const int L = secular_lanes(K, merges);
+        const int rpw = kWave / L;  // roots per wave
+        if (slot * rpw >= K) continue;
+        int s, sz;
+        node_geom(n, depth, merge, &s, &sz);
+        const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
+        int iters = 0;
+        for (int grp = slot; grp * rpw < K; grp += slots)
+            iters += secular_group(n, s, sz, K, L, grp, rho, ad, az, adz, rlam, rs, rn1, rn2,
+                                   info);
+        for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
+        if (lane == 0 && iters) atomicAdd(&counts_rw[4 * merge + 3], iters);
+    }
 }
 
 // Merge step 4: product-form update vector (dlaed9_br.f:218-266): one lane per
@@ -895,11 +930,13 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     const int wa = max(j0, g0 - 1), wb = min(j1, g0 + kGroup);
     auto term = [&](double2 r, double dj, int j, bool check) {
         double dl = (di - r.x) - r.y;
-        if (check) {
+        if (check) {  // j == i and the pole-adjacent roots lie in the window only
             if (i == j) dl = rn1[s + j];
             else if (i == j + 1) dl = rn2[s + j];
+            prod *= (j == i) ? dl : dl * rcp_nr(di - dj);
+        } else {
+            prod *= dl * rcp_nr(di - dj);
         }
-        prod *= (j == i) ? dl : dl * rcp_nr(di - dj);
     };
     auto range = [&](int ja, int jb, bool check) {
         int j = ja;

@@ -1,0 +1,45 @@
+// brdc_launch.h -- launch-shape helpers of the secular kernel, shared by the
+// host enqueue code and the device (no kernels here, so host-only builds can
+// include it).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace brdc {
+
+constexpr int kWave = 64;
+
+// lanes per root: about 4+ poles per lane per pass, and no more lanes than
+// the level needs to fill the machine (~kTargetLanes lanes in flight)
+constexpr int kTargetLanes = 1 << 19;
+__host__ __device__ inline int secular_lanes(int K, int merges) {
+    int L = 1;
+    while (L < kWave && 4 * 2 * L <= K && (long long)merges * K * (2 * L) <= kTargetLanes) L <<= 1;
+    return L;
+}
+
+// waves a merge of K poles needs at L lanes per root
+__host__ __device__ inline int secular_waves(int K, int merges) {
+    const int L = secular_lanes(K, merges);
+    return (K * L + kWave - 1) / kWave;
+}
+
+// The most waves any merge of K <= maxsz poles needs.  secular_waves is not
+// monotone in K (a smaller K can be given more lanes per root), but it grows
+// with K while the lane count holds, so its maximum sits at the upper end of
+// a constant-lane run: K = maxsz, K = 8L - 1 (the next K doubles L) or
+// K = kTargetLanes / (merges * L) (the next K halves L).
+inline int secular_slots(int maxsz, int merges) {
+    int best = secular_waves(maxsz, merges);
+    for (int L = 1; L <= kWave; L <<= 1) {
+        const long long cand[2] = {8LL * L - 1, kTargetLanes / ((long long)merges * L)};
+        for (long long k : cand)
+            if (k >= 1 && k <= maxsz) {
+                const int w = secular_waves((int)k, merges);
+                if (w > best) best = w;
+            }
+    }
+    return best;
+}
+
+}  // namespace brdc

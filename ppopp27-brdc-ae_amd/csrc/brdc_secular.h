@@ -39,6 +39,32 @@ __host__ __device__ __forceinline__ double dev_max3(double a, double b, double c
     return fmax(fabs(a), fmax(fabs(b), fabs(c)));
 }
 
+// Reciprocal for the pole loops: hardware v_rcp_f64 refined by two Newton
+// steps (error below one ulp); the host build of this header uses 1/x.
+__host__ __device__ __forceinline__ double rcp_nr(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// Division in the per-root step logic: a * rcp_nr(b) on the device (within
+// one ulp of a/b, about half the latency of the IEEE division sequence).  It
+// only perturbs the iteration path; the stopping test and the converged root
+// keep their accuracy.  The host build divides exactly.
+__host__ __device__ __forceinline__ double fdiv(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__) || defined(BRDC_HOST_DEVICE_ARITH)
+    return a * rcp_nr(b);
+#else
+    return a / b;
+#endif
+}
+
 // Two-pole closed form (the DLAED5 role).  vec gets the normalised eigenvector.
 __host__ __device__ inline void secular_two(int i, const double* d, const double* z, double rho,
                                    double* lam, double* v0, double* v1, double* tau_out) {
@@ -111,32 +137,32 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
     if (kniter == 2) {
         if (orgati) {
             temp = (dd[2] - dd[1]) / 2.0;
-            c = rho + zz[0] / ((dd[0] - dd[1]) - temp);
+            c = rho + fdiv(zz[0], (dd[0] - dd[1]) - temp);
             a = c * (dd[1] + dd[2]) + zz[1] + zz[2];
             b = c * dd[1] * dd[2] + zz[1] * dd[2] + zz[2] * dd[1];
         } else {
             temp = (dd[0] - dd[1]) / 2.0;
-            c = rho + zz[2] / ((dd[2] - dd[1]) - temp);
+            c = rho + fdiv(zz[2], (dd[2] - dd[1]) - temp);
             a = c * (dd[0] + dd[1]) + zz[0] + zz[1];
             b = c * dd[0] * dd[1] + zz[0] * dd[1] + zz[1] * dd[0];
         }
-        temp = dev_max3(a, b, c);
-        a /= temp;
-        b /= temp;
-        c /= temp;
+        temp = rcp_nr(dev_max3(a, b, c));
+        a *= temp;
+        b *= temp;
+        c *= temp;
         if (c == 0.0)
-            tau = b / a;
+            tau = fdiv(b, a);
         else if (a <= 0.0)
-            tau = (a - sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
+            tau = fdiv(a - sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
         else
-            tau = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
+            tau = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
         if (tau < lbd || tau > ubd)
             tau = (lbd + ubd) / 2.0;
         if (dd[0] == tau || dd[1] == tau || dd[2] == tau) {
             tau = 0.0;
         } else {
-            temp = finit + tau * zz[0] / (dd[0] * (dd[0] - tau)) +
-                   tau * zz[1] / (dd[1] * (dd[1] - tau)) + tau * zz[2] / (dd[2] * (dd[2] - tau));
+            temp = finit + fdiv(tau * zz[0], dd[0] * (dd[0] - tau)) +
+                   fdiv(tau * zz[1], dd[1] * (dd[1] - tau)) + fdiv(tau * zz[2], dd[2] * (dd[2] - tau));
             if (temp <= 0.0)
                 lbd = tau;
             else
@@ -176,9 +202,9 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
     fc = df = ddf = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double r = 1.0 / (ds[k] - tau);
+        const double r = rcp_nr(ds[k] - tau);
         const double t1 = zs[k] * r, t2 = t1 * r, t3 = t2 * r;
-        fc += t1 / ds[k];
+        fc += fdiv(t1, ds[k]);
         df += t2;
         ddf += t3;
     }
@@ -201,18 +227,18 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
         a = (t1 + t2) * f - t1 * t2 * df;
         b = t1 * t2 * f;
         c = f - (t1 + t2) * df + t1 * t2 * ddf;
-        temp = dev_max3(a, b, c);
-        a /= temp;
-        b /= temp;
-        c /= temp;
+        temp = rcp_nr(dev_max3(a, b, c));
+        a *= temp;
+        b *= temp;
+        c *= temp;
         if (c == 0.0)
-            eta = b / a;
+            eta = fdiv(b, a);
         else if (a <= 0.0)
-            eta = (a - sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
+            eta = fdiv(a - sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
         else
-            eta = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
+            eta = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
         if (f * eta >= 0.0)
-            eta = -f / df;
+            eta = -fdiv(f, df);
         tau += eta;
         if (tau < lbd || tau > ubd)
             tau = (lbd + ubd) / 2.0;
@@ -226,8 +252,8 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
             goto done;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const double r = 1.0 / (ds[k] - tau);
-            const double u1 = zs[k] * r, u2 = u1 * r, u3 = u2 * r, u4 = u1 / ds[k];
+            const double r = rcp_nr(ds[k] - tau);
+            const double u1 = zs[k] * r, u2 = u1 * r, u3 = u2 * r, u4 = fdiv(u1, ds[k]);
             fc += u4;
             erretm += fabs(u4);
             df += u2;
@@ -248,20 +274,6 @@ done:
         tau *= sclinv;
     *tau_out = tau;
     return info;
-}
-
-// Reciprocal for the pole loops: hardware v_rcp_f64 refined by two Newton
-// steps (error below one ulp); the host build of this header uses 1/x.
-__host__ __device__ __forceinline__ double rcp_nr(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-#else
-    return 1.0 / x;
-#endif
 }
 
 // One evaluation pass over the poles j outside [ex_lo, ex_lo + nex), with
@@ -311,7 +323,7 @@ struct SecularSolve {
         K = K_;
         i = i_;
         rho = rho_;
-        rhoinv = 1.0 / rho;
+        rhoinv = fdiv(1.0, rho);
         last = (i == K - 1);
         out.info = 0;
         out.iters = 0;
@@ -322,7 +334,7 @@ struct SecularSolve {
             dip1 = E.d(K - 2); // d_{K-2}
             zi2 = E.zsq(K - 1);
             zip12 = E.zsq(K - 2);
-            midpt = rho / 2.0;
+            midpt = rho * 0.5;
             ev_org = di;
             ev_tau = midpt;
             ev_exlo = K - 2;
@@ -333,7 +345,7 @@ struct SecularSolve {
             zi2 = E.zsq(i);
             zip12 = E.zsq(i + 1);
             del = dip1 - di;
-            midpt = del / 2.0;
+            midpt = del * 0.5;
             ev_org = di;
             ev_tau = midpt;
             ev_exlo = i;
@@ -369,8 +381,8 @@ struct SecularSolve {
             dpsi = s.dpsi + s.dphi;
             const double err = s.J - (double)(K - 1) * s.S;
             const double dn = -tau;
-            phi = zi2 / dn;
-            dphi = phi / dn;
+            phi = fdiv(zi2, dn);
+            dphi = fdiv(phi, dn);
             erretm = 8.0 * (-phi - psi) + err - phi + rhoinv + fabs(tau) * (dpsi + dphi);
             w = rhoinv + phi + psi;
         } else {
@@ -380,8 +392,8 @@ struct SecularSolve {
             dphi = s.dphi;
             const double err = s.J - (double)ii * s.S;
             const double dii = -tau;
-            const double temp = zii2 / dii;
-            dw = dpsi + dphi + temp / dii;
+            const double temp = fdiv(zii2, dii);
+            dw = dpsi + dphi + fdiv(temp, dii);
             w0 = rhoinv + S;
             w = w0 + temp;
             erretm = 8.0 * A + err + 2.0 * rhoinv + 3.0 * fabs(temp) + (k >= 2 ? fabs(tau) : 0.0) * dw;
@@ -415,24 +427,24 @@ struct SecularSolve {
             const double b = dn1 * dn * w;
             if (c < 0.0) c = fabs(c);
             if (c == 0.0)
-                eta = -w / (dpsi + dphi);
+                eta = -fdiv(w, dpsi + dphi);
             else if (a >= 0.0)
-                eta = (a + sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
+                eta = fdiv(a + sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
             else
-                eta = 2.0 * b / (a - sqrt(fabs(a * a - 4.0 * b * c)));
-            if (w * eta > 0.0) eta = -w / (dpsi + dphi);
+                eta = fdiv(2.0 * b, a - sqrt(fabs(a * a - 4.0 * b * c)));
+            if (w * eta > 0.0) eta = -fdiv(w, dpsi + dphi);
         } else {
             const double dl_i = delta_at(di), dl_ip1 = delta_at(dip1);
             if (!swtch3) {
                 double c;
                 if (niter == 2 || !swtch) {
                     if (orgati)
-                        c = w - dl_ip1 * dw - (di - dip1) * (zi2 / (dl_i * dl_i));
+                        c = w - dl_ip1 * dw - (di - dip1) * fdiv(zi2, dl_i * dl_i);
                     else
-                        c = w - dl_i * dw - (dip1 - di) * (zip12 / (dl_ip1 * dl_ip1));
+                        c = w - dl_i * dw - (dip1 - di) * fdiv(zip12, dl_ip1 * dl_ip1);
                 } else {
                     const double dlo = delta_at(dorg);
-                    const double temp = zii2 / (dlo * dlo);
+                    const double temp = fdiv(zii2, dlo * dlo);
                     if (orgati)
                         dpsi += temp;
                     else
@@ -452,11 +464,11 @@ struct SecularSolve {
                             a = dl_i * dl_i * dpsi + dl_ip1 * dl_ip1 * dphi;
                         }
                     }
-                    eta = b / a;
+                    eta = fdiv(b, a);
                 } else if (a <= 0.0) {
-                    eta = (a - sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
+                    eta = fdiv(a - sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
                 } else {
-                    eta = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
+                    eta = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
                 }
             } else {
                 const double dl_m = delta_at(dm1), dl_p = delta_at(dp1);
@@ -467,12 +479,12 @@ struct SecularSolve {
                     zz0 = dl_m * dl_m * dpsi;
                     zz2 = dl_p * dl_p * dphi;
                 } else if (orgati) {
-                    const double t1 = zm12 / (dl_m * dl_m);
+                    const double t1 = fdiv(zm12, dl_m * dl_m);
                     c = temp - dl_p * (dpsi + dphi) - (dm1 - dp1) * t1;
                     zz0 = zm12;
                     zz2 = dl_p * dl_p * ((dpsi - t1) + dphi);
                 } else {
-                    const double t1 = zp12 / (dl_p * dl_p);
+                    const double t1 = fdiv(zp12, dl_p * dl_p);
                     c = temp - dl_m * (dpsi + dphi) - (dp1 - dm1) * t1;
                     zz0 = dl_m * dl_m * (dpsi + (dphi - t1));
                     zz2 = zp12;
@@ -484,11 +496,11 @@ struct SecularSolve {
                     return true;
                 }
             }
-            if (w * eta >= 0.0) eta = -w / dw;
+            if (w * eta >= 0.0) eta = -fdiv(w, dw);
         }
         const double temp = tau + eta;
         if (temp > dltub || temp < dltlb)
-            eta = (w < 0.0) ? (dltub - tau) / 2.0 : (dltlb - tau) / 2.0;
+            eta = (w < 0.0) ? (dltub - tau) * 0.5 : (dltlb - tau) * 0.5;
         prew = w;
         tau += eta;
         k++;
@@ -500,25 +512,25 @@ struct SecularSolve {
         if (last) {
             const double c = rhoinv + s0.S;
             const double dl_ii = (dip1 - di) - midpt, dl_n = -midpt;
-            const double w_ = c + zip12 / dl_ii + zi2 / dl_n;
+            const double w_ = c + fdiv(zip12, dl_ii) + fdiv(zi2, dl_n);
             const double dl = di - dip1;
             const double a = -c * dl + zip12 + zi2;
             const double b = zi2 * dl;
             if (w_ <= 0.0) {
-                const double temp = zip12 / (di - dip1 + rho) + zi2 / rho;
+                const double temp = fdiv(zip12, di - dip1 + rho) + fdiv(zi2, rho);
                 if (c <= temp)
                     tau = rho;
                 else if (a < 0.0)
-                    tau = 2.0 * b / (sqrt(a * a + 4.0 * b * c) - a);
+                    tau = fdiv(2.0 * b, sqrt(a * a + 4.0 * b * c) - a);
                 else
-                    tau = (a + sqrt(a * a + 4.0 * b * c)) / (2.0 * c);
+                    tau = fdiv(a + sqrt(a * a + 4.0 * b * c), 2.0 * c);
                 dltlb = midpt;
                 dltub = rho;
             } else {
                 if (a < 0.0)
-                    tau = 2.0 * b / (sqrt(a * a + 4.0 * b * c) - a);
+                    tau = fdiv(2.0 * b, sqrt(a * a + 4.0 * b * c) - a);
                 else
-                    tau = (a + sqrt(a * a + 4.0 * b * c)) / (2.0 * c);
+                    tau = fdiv(a + sqrt(a * a + 4.0 * b * c), 2.0 * c);
                 dltlb = 0.0;
                 dltub = midpt;
             }
@@ -531,15 +543,15 @@ struct SecularSolve {
             return false;
         }
         const double c = rhoinv + s0.S;
-        const double w_ = c + zi2 / (-midpt) + zip12 / (del - midpt);
+        const double w_ = c + fdiv(zi2, -midpt) + fdiv(zip12, del - midpt);
         if (w_ > 0.0) {
             orgati = true;
             const double a = c * del + zi2 + zip12;
             const double b = zi2 * del;
             if (a > 0.0)
-                tau = 2.0 * b / (a + sqrt(fabs(a * a - 4.0 * b * c)));
+                tau = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
             else
-                tau = (a - sqrt(fabs(a * a - 4.0 * b * c))) / (2.0 * c);
+                tau = fdiv(a - sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
             dltlb = 0.0;
             dltub = midpt;
         } else {
@@ -547,9 +559,9 @@ struct SecularSolve {
             const double a = c * del - zi2 - zip12;
             const double b = zip12 * del;
             if (a < 0.0)
-                tau = 2.0 * b / (a - sqrt(fabs(a * a + 4.0 * b * c)));
+                tau = fdiv(2.0 * b, a - sqrt(fabs(a * a + 4.0 * b * c)));
             else
-                tau = -(a + sqrt(fabs(a * a + 4.0 * b * c))) / (2.0 * c);
+                tau = -fdiv(a + sqrt(fabs(a * a + 4.0 * b * c)), 2.0 * c);
             dltlb = -midpt;
             dltub = 0.0;
         }

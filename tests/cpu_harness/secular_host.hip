@@ -59,3 +59,31 @@ extern "C" int secular_host(int K, int i, const double* d, const double* z, doub
     out[5] = r.iters;
     return r.info;
 }
+
+// All K roots of one merge; returns the number of roots that did not converge
+// and writes each root's (lambda, tau, near1, near2, org, iters) to out[6*i].
+extern "C" int secular_host_all(int K, const double* d, const double* z, double rho, double* out) {
+    int bad = 0;
+    for (int i = 0; i < K; i++) bad += (secular_host(K, i, d, z, rho, out + 6 * i) != 0);
+    return bad;
+}
+
+// Wave-slot bound of the secular launch: for every depth of an order-n tree
+// with the given leaf size, count the K <= max node size whose wave need
+// exceeds the slots the launch provides.  The kernel stays correct either
+// way (a task walks all root groups of its merge); this checks that the
+// bound is tight so no wave does two groups' work.
+#include "../../ppopp27-brdc-ae_amd/csrc/brdc_launch.h"
+#include "../../ppopp27-brdc-ae_amd/csrc/brdc_tree.h"
+
+extern "C" int secular_slot_violations(int n, int leaf) {
+    int bad = 0;
+    const int levels = brdc::tree_levels(n, leaf);
+    for (int depth = 0; depth < levels; depth++) {
+        const int merges = 1 << depth;
+        const int maxsz = brdc::max_node_size(n, depth);
+        const int slots = brdc::secular_slots(maxsz, merges);
+        for (int K = 1; K <= maxsz; K++) bad += (brdc::secular_waves(K, merges) > slots);
+    }
+    return bad;
+}

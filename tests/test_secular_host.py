@@ -67,3 +67,14 @@ def test_device_secular_matches_oracle_on_merge_like_inputs(sec, orc):
             # both stop at |w| <= eps*erretm; the device recomputes delta from
             # (org, tau) instead of accumulating it, so roots agree to a few ulps
             assert abs(out[0] - lam) <= 1e-14 * max(1.0, abs(lam))
+
+
+def test_secular_launch_slots_cover_every_merge(sec):
+    """The secular launch gives each merge as many wave slots as the largest
+    need of any merge size up to the level's maximum (the need is not monotone
+    in the size), at every level of several tree shapes."""
+    sec.secular_slot_violations.argtypes = [ctypes.c_int, ctypes.c_int]
+    sec.secular_slot_violations.restype = ctypes.c_int
+    for n in (1000, 3000, 16384, 49152, 100000, 262144):
+        for leaf in (8, 16, 32, 64):
+            assert sec.secular_slot_violations(n, leaf) == 0, (n, leaf)
