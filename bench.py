@@ -41,6 +41,21 @@ def invariants_err(w, d, e):
     return max(tr, fr)
 
 
+def profiled_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a plain
+    copy's counts), or (None, None) when no summary covers it."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" \
+                        and parts[1].split("<")[0] == "k_" + kernel_class:
+                    return float(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(mats, budget_s=25.0):
     """Time the CPU oracle (restatement of the reference BR, OpenMP) on the
     host cores.  Returns (record, eigenvalues per generator)."""
@@ -85,6 +100,19 @@ def cpu_baseline(mats, budget_s=25.0):
     return rec, eigs
 
 
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +123,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying a captured graph")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB, three "
+                         "times (calibrates the HBM byte counters of a profiler run)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU layout of this path is a design note, not a measured run",
@@ -221,20 +255,24 @@ def main():
     dom = max(prof["kernels"].items(), key=lambda kv: kv[1]["ms"])
     dom_name, dom_rec = dom
     roof = None
+    traffic, traffic_src = profiled_traffic(dom_name)
     if dom_name in prof["flops"]:
         flops = prof["flops"][dom_name]
         achieved = flops / (dom_rec["ms"] * 1e-3)
         roof = {"bound": "mfma", "pipe": "FP64 VALU (no matrix-core op on this path; the FP64 "
                 "vector and FP64 matrix peaks of MI355X are both 78.6 TFLOP/s)",
                 "kernel": dom_name, "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": traffic,
+                "traffic_unit": "bytes/launch", "traffic_source": traffic_src,
+                "algorithmic_flops_per_launch": flops / dom_rec["launches"],
                 "launches": dom_rec["launches"], "avg_launch_us": dom_rec["avg_us"]}
     else:
         bytes_ = prof["bytes"].get(dom_name)
         if bytes_:
             achieved = bytes_ / (dom_rec["ms"] * 1e-3)
             roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9, "peak": 8000.0,
-                    "unit": "GB/s", "frac": achieved / 8e12, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / 8e12, "traffic": traffic,
+                    "traffic_source": traffic_src,
                     "launches": dom_rec["launches"], "avg_launch_us": dom_rec["avg_us"]}
 
     line = {

@@ -25,7 +25,7 @@
  *           Reinsch, Wilkinson, Numer. Math. 11, 1968), eigenvalues sorted
  *           ascending; only the first and last eigenvector rows are kept.
  *   DLAMRG  two-list merge permutation.
- * Pinned by tests/test_oracle_*.py against scipy's OpenBLAS (dsterf,
+ * Pinned by tests/test_oracle.py and tests/test_secular_host.py against scipy's OpenBLAS (dsterf,
  * dlaed4/5/6 golden vectors), dense eigh, closed-form spectra and invariants.
  */
 #include "brdc_oracle.h"

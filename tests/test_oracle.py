@@ -222,3 +222,34 @@ def test_splits_zero_matrix_and_small(orc):
         w, info = orc.dstebr(d, e)
         T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
         assert info == 0 and rel_err(w, np.linalg.eigvalsh(T)) < 1e-14
+
+
+def test_sterf_matches_dsterf_fixtures(orc):
+    """The oracle's values-only QL (the DSTERF role) against LAPACK DSTERF."""
+    g = np.load(os.path.join(GOLD, "dsterf_eigs.npz"))
+    for kind in GENERATORS:
+        for n in (26, 300, 1000):
+            d, e = generate(kind, n, seed_for(1, n))
+            w, info = orc.sterf(d, e)
+            assert info == 0 and rel_err(w, g[f"{kind}_{n}"]) <= 1e-13, (kind, n)
+
+
+def test_lamrg_brute_force(orc):
+    """DLAMRG: two sorted runs (ascending for stride +1, descending for -1)
+    merged into one ascending index list; ties go to the first run."""
+    rng = np.random.default_rng(5)
+    for n1, n2 in ((0, 3), (3, 0), (1, 1), (5, 7), (40, 33)):
+        for s1 in (1, -1):
+            for s2 in (1, -1):
+                a1 = np.sort(rng.integers(0, 6, n1).astype(float))
+                a2 = np.sort(rng.integers(0, 6, n2).astype(float))
+                a = np.concatenate([a1 if s1 > 0 else a1[::-1], a2 if s2 > 0 else a2[::-1]])
+                idx = orc.lamrg(a, n1, n2, s1, s2)
+                assert sorted(idx.tolist()) == list(range(n1 + n2))
+                assert np.all(np.diff(a[idx]) >= 0)
+                # ties: within equal values, every first-run index precedes
+                # every second-run index
+                for v in np.unique(a):
+                    pos = [p for p in idx if a[p] == v]
+                    side = [p >= n1 for p in pos]
+                    assert side == sorted(side)

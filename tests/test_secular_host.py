@@ -78,3 +78,32 @@ def test_secular_launch_slots_cover_every_merge(sec):
     for n in (1000, 3000, 16384, 49152, 100000, 262144):
         for leaf in (8, 16, 32, 64):
             assert sec.secular_slot_violations(n, leaf) == 0, (n, leaf)
+
+
+@pytest.mark.parametrize("kind", ["normal", "uniform", "clustered", "toeplitz"])
+def test_device_secular_on_every_merge_of_a_solve(sec, orc, kind):
+    """Every secular problem the oracle's tree meets on a seeded n = 3000
+    matrix (leaf 16, the device default): the device root finder converges
+    on each root and agrees with the oracle's DLAED4."""
+    from brdc.generators import generate, seed_for
+    n = 3000
+    d, e = generate(kind, n, seed_for(2, n))
+    probs = orc.merge_problems(d, e, smlsiz=16)
+    assert len(probs) == 2 ** 8 - 1
+    checked = 0
+    rng = np.random.default_rng(3)
+    for (level, start, rho, dl, w) in probs:
+        K = dl.shape[0]
+        if K == 0:
+            continue
+        # every root of the big merges would be O(K^2) ctypes calls: sample
+        roots = range(K) if K <= 64 else sorted(set(rng.integers(0, K, 48).tolist()) | {0, K - 1})
+        for i in roots:
+            info, out = run(sec, dl, w, rho, i)
+            assert info == 0, (level, start, K, i)
+            lam, _, info_ref = orc.laed4(dl, w, rho, i)
+            assert info_ref == 0
+            scale = max(abs(lam), np.max(np.abs(dl)))
+            assert abs(out[0] - lam) <= 1e-14 * scale, (level, start, K, i)
+            checked += 1
+    assert checked > 100
