@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <climits>
 #include <vector>
 
 #include "../../include/brdc.h"
@@ -24,6 +25,27 @@ namespace {
 constexpr int kDefaultLeaf = 16;
 constexpr int kSecWaves = 4;       // waves per block of the secular kernel (independent waves)
 constexpr int kSecMaxWaves = 3072;  // secular grid cap: about 3 resident waves per SIMD
+constexpr int kStreamWaves = 8;     // waves per workgroup of the streamed secular kernel
+constexpr int kStreamMinK = 2048;   // merges with at least this many poles are streamed
+
+// Workgroups of the streamed secular kernel that fit the device at once
+// (CUs x resident groups per CU), cached per device.
+int stream_groups() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] > 0) return cached[dev];
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_secular_stream<kStreamWaves>,
+                                                     kWave * kStreamWaves, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    cached[dev] = cus * per_cu;
+    return cached[dev];
+}
 constexpr size_t kAlign = 256;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
@@ -88,7 +110,7 @@ Layout make_layout(int n, int leaf) {
     L.P = take(ni);
     L.rend = take(ni);
     L.fl = take(ni);
-    L.counts = take((size_t)4 * L.merges * sizeof(int));
+    L.counts = take((size_t)kCountStride * L.merges * sizeof(int));
     L.info = take(sizeof(int));
     L.total = off;
     return L;
@@ -236,7 +258,7 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
         // the O(K^2) kernels: one block per 64 roots/poles, WL waves wide
         const int bpg = (maxsz + kGroup - 1) / kGroup;
         const int WL = waves_for(maxsz, kMaxWaves);
-        int* counts = counts_all + 4 * (merges - 1);
+        int* counts = counts_all + kCountStride * (merges - 1);
         const int nxt = cur ^ 1;
         const int root = (depth == 0);
 
@@ -252,19 +274,32 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
                            adz, aq, gd, gq1, gq2, td, tq1, tq2, counts);
         }
         BRDC_LAUNCH_CHECK();
+        // merges of K >= kStreamMinK poles go to the streamed kernel, the
+        // rest to the lanes-per-root kernel
+        const bool streamed = (maxsz >= kStreamMinK);
         {
             ProfScope ps(kSecular, st);
-            // wave slots per merge: the most any K <= maxsz can need; the grid
-            // is capped and walks (merge, slot) tasks
-            const int slots = secular_slots(maxsz, merges);
+            // wave slots per merge: the most any K it handles can need; the
+            // grid is capped and walks (merge, slot) tasks
+            const int slots = secular_slots(std::min(maxsz, kStreamMinK - 1), merges);
             const int waves = std::min(merges * slots, kSecMaxWaves);
             const int blocks = (waves + kSecWaves - 1) / kSecWaves;
             hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves), 0, st,
-                               n, depth, slots, (const double*)ew, (const int*)counts,
-                               (const double*)ad, (const double*)az, (const double2*)adz, rlam,
-                               rs, rn1, rn2, counts, info);
+                               n, depth, slots, streamed ? kStreamMinK : INT_MAX,
+                               (const double*)ew, (const int*)counts, (const double*)ad,
+                               (const double*)az, (const double2*)adz, rlam, rs, rn1, rn2, counts,
+                               info);
         }
         BRDC_LAUNCH_CHECK();
+        if (streamed) {
+            ProfScope ps(kSecular, st);
+            const int groups = std::min(stream_groups(), (n + kWave - 1) / kWave);
+            hipLaunchKernelGGL(k_secular_stream<kStreamWaves>, dim3(groups),
+                               dim3(kWave * kStreamWaves), 0, st, n, depth, kStreamMinK,
+                               (const double*)ew, (const int*)counts, (const double2*)adz, rlam,
+                               rs, rn1, rn2, counts, info);
+            BRDC_LAUNCH_CHECK();
+        }
         if (!root) {
             {
                 ProfScope ps(kWprod, st);
@@ -354,7 +389,7 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
     if (workspace_bytes < L.total) return -BRDC_STATUS_INVALID_WORKSPACE;
     const int total = (n > 1) ? (1 << L.levels) - 1 : 0;
     if (total == 0) return 0;
-    std::vector<int> counts((size_t)4 * total);
+    std::vector<int> counts((size_t)kCountStride * total);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (hipMemcpyAsync(counts.data(),
                        static_cast<const char*>(workspace) + L.counts,
@@ -367,7 +402,7 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
             if (rec >= max_merges) return total;
             int s, sz;
             node_geom(n, depth, i, &s, &sz);
-            const int* c = &counts[(size_t)4 * ((1 << depth) - 1 + i)];
+            const int* c = &counts[(size_t)kCountStride * ((1 << depth) - 1 + i)];
             int* o = out_host + 6 * rec;
             o[0] = depth;
             o[1] = s;

@@ -12,7 +12,8 @@
 //     r*   per root: lambda, tau, near1, near2 (+ int org)          [secular]
 //     w    product-form update vector                               [wprod]
 //     qo*  new boundary rows of the roots                           [rows]
-//   counts: 4 ints per merge (K, G, T, secular evaluations)
+//   counts: kCountStride ints per merge (K, G, T, secular evaluations,
+//           root-claim counter)
 // Every kernel derives its merge geometry from (n, depth, merge index), so a
 // level is a fixed launch sequence with no host round trip.
 #pragma once
@@ -698,10 +699,11 @@ __global__ __launch_bounds__(T) void k_deflate(
         toff += __popcll(bt);
     }
     if (tid == 0) {
-        counts[4 * merge + 0] = K;
-        counts[4 * merge + 1] = G;
-        counts[4 * merge + 2] = Tn;
-        counts[4 * merge + 3] = 0;
+        counts[kCountStride * merge + 0] = K;
+        counts[kCountStride * merge + 1] = G;
+        counts[kCountStride * merge + 2] = Tn;
+        counts[kCountStride * merge + 3] = 0;
+        counts[kCountStride * merge + 4] = 0;  // root-claim counter of the streamed secular kernel
     }
 }
 
@@ -865,13 +867,13 @@ __device__ __forceinline__ int secular_group(int n, int s, int sz, int K, int L,
     return iters;
 }
 
-// All roots of all merges of one level.  Work is split into tasks (merge,
-// slot), slot < slots; the task's wave solves root groups slot, slot + slots, ... of its
+// All roots of the level's merges with K < kstream.  Work is split into
+// tasks (merge, slot), slot < slots; the task's wave solves root groups slot, slot + slots, ... of its
 // merge, so every group is covered whatever the slot count, and the grid is
 // capped, each wave walking tasks with a grid stride.
 template <int WL>
 __global__ __launch_bounds__(kWave * WL) void k_secular(
-    int n, int depth, int slots, const double* __restrict__ ew, const int* counts,
+    int n, int depth, int slots, int kstream, const double* __restrict__ ew, const int* counts,
     const double* __restrict__ ad, const double* __restrict__ az,
     const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
     double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
@@ -882,10 +884,10 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
     for (int task = uniform(blockIdx.x * WL + (int)(threadIdx.x / kWave)); task < ntask;
          task += nwave) {
         const int merge = task / slots, slot = task - merge * slots;
-        const int K = uniform(counts[4 * merge]);
+        const int K = uniform(counts[kCountStride * merge]);
         const int L = secular_lanes(K, merges);
         const int rpw = kWave / L;  // roots per wave
-        if (slot * rpw >= K) continue;
+        if (slot * rpw >= K || K >= kstream) continue;  // idle slot / streamed merge
         int s, sz;
         node_geom(n, depth, merge, &s, &sz);
         const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
@@ -894,7 +896,255 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
             iters += secular_group(n, s, sz, K, L, grp, rho, ad, az, adz, rlam, rs, rn1, rn2,
                                    info);
         for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
-        if (lane == 0 && iters) atomicAdd(&counts_rw[4 * merge + 3], iters);
+        if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
+    }
+}
+
+// Streamed secular solve for the merges with K >= kstream (the big merges
+// of the top levels, where the O(K^2) work sits).
+//
+// A workgroup of W waves serves 64/L roots at a time, L lanes per root, the
+// same root positions in every wave; the wave w streams the pole slice
+// [K*w/W, K*(w+1)/W) and lane c of a root takes every L-th pole of it.
+// Every lane of a wave needs the same few poles at the same time: the wave
+// fetches 128 poles with two coalesced loads and the lanes read them from
+// LDS (L distinct addresses per read, broadcast within each root's lanes).
+// The waves' partial sums meet in LDS; wave 0 alone holds the roots' state
+// machines, advances them and publishes the next pass's evaluation points,
+// so a pass costs two barriers and one state-machine step.  A root group
+// whose root has converged takes the next unclaimed root of the merge
+// (per-merge atomic counter), so no lane idles while the merge has roots
+// left.  L is chosen per merge so that each root position works through
+// several roots (the last root's passes are a small tail of the total) while
+// each lane still streams a long run of poles per pass.
+//
+// The split of a pass's poles into the psi side (below a root: delta < 0)
+// and the phi side is per root; all roots' excluded poles lie in a window
+// [wlo, whi) of the merge, so only the poles inside it need per-lane tests.
+constexpr int kStreamBatch = 8;
+constexpr int kStageChunk = 128;  // poles staged per chunk (two per lane)
+
+struct StreamAcc {  // one side's sums over a pole range
+    double S, D, J;
+};
+
+// Reciprocal of the streamed pole terms (the poles outside every root's
+// exclusion window): v_rcp_f64 and one Newton step, relative error below
+// 2^-50 (a few ulp).  The stopping bound of each pass, eps * erretm with
+// erretm >= 8 A + sum_j |j - ii| |term_j| >= 9 A over these poles (A = sum of
+// |term_j|), stays above that rounding, and with K >= kstream poles the
+// index-weighted part is many times A.
+__device__ __forceinline__ double rcp_stream(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// Poles [ja, jb) pass through this wave's LDS slot in chunks of 128: lane l
+// fetches poles c0 + l and c0 + 64 + l (the next chunk's loads are in flight
+// while the current chunk is consumed).  One wave's LDS operations complete
+// in order, so the slot needs no barrier.
+template <class F>
+__device__ __forceinline__ void staged_range(const double2* __restrict__ dz, double2* slot, int ja,
+                                             int jb, F&& consume) {
+    if (ja >= jb) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const double2 zero = make_double2(0.0, 0.0);
+    double2 n0 = (ja + lane < jb) ? dz[ja + lane] : zero;
+    double2 n1 = (ja + kWave + lane < jb) ? dz[ja + kWave + lane] : zero;
+    for (int c0 = ja; c0 < jb; c0 += kStageChunk) {
+        slot[lane] = n0;
+        slot[kWave + lane] = n1;
+        const int c1 = c0 + kStageChunk;
+        if (c1 < jb) {
+            n0 = (c1 + lane < jb) ? dz[c1 + lane] : zero;
+            n1 = (c1 + kWave + lane < jb) ? dz[c1 + kWave + lane] : zero;
+        }
+        consume(c0, min(kStageChunk, jb - c0));
+    }
+}
+
+// Poles [ja, jb) that lie wholly on one side of every root; this lane (c of
+// its root's L lanes) takes poles ja + c, ja + c + L, ...
+__device__ __forceinline__ void stream_side(const double2* __restrict__ dz, double2* slot, int ja,
+                                            int jb, int c, int L, double org, double tau,
+                                            StreamAcc& a) {
+    const double dL = (double)L;
+    staged_range(dz, slot, ja, jb, [&](int c0, int nb) {
+        const int nfull = nb / (kStreamBatch * L);  // batches every lane has in full
+        int o = c;
+        for (int k = 0; k < nfull; k++, o += kStreamBatch * L) {
+            double2 p[kStreamBatch];
+#pragma unroll
+            for (int b = 0; b < kStreamBatch; b++) p[b] = slot[o + b * L];
+            // J gets sum_b (j + b L) t_b = j * T + L * sum_b b t_b
+            double T = 0.0, B = 0.0;
+#pragma unroll
+            for (int b = 0; b < kStreamBatch; b++) {
+                const double r = rcp_stream((p[b].x - org) - tau);
+                const double t = p[b].y * r;
+                T += t;
+                a.D = fma(t, r, a.D);
+                if (b > 0) B = fma((double)b, t, B);
+            }
+            a.S += T;
+            a.J = fma((double)(c0 + o), T, a.J);
+            a.J = fma(dL, B, a.J);
+        }
+        for (; o < nb; o += L) {
+            const double2 p = slot[o];
+            const double r = rcp_stream((p.x - org) - tau);
+            const double t = p.y * r;
+            a.S += t;
+            a.D = fma(t, r, a.D);
+            a.J = fma((double)(c0 + o), t, a.J);
+        }
+    });
+}
+
+// Poles [ja, jb) inside the window: each root skips its excluded poles
+// [exlo, exlo + nex) and files the others by side.
+__device__ __forceinline__ void stream_window(const double2* __restrict__ dz, double2* slot,
+                                              int ja, int jb, int c, int L, double org,
+                                              double tau, int exlo, int nex, StreamAcc& lo,
+                                              StreamAcc& hi) {
+    staged_range(dz, slot, ja, jb, [&](int c0, int nb) {
+        for (int o = c; o < nb; o += L) {
+            const int j = c0 + o;
+            const double2 p = slot[o];
+            if ((unsigned)(j - exlo) < (unsigned)nex) continue;
+            const double r = rcp_nr((p.x - org) - tau);
+            const double t = p.y * r;
+            StreamAcc& a = (j < exlo) ? lo : hi;
+            a.S += t;
+            a.D = fma(t, r, a.D);
+            a.J = fma((double)j, t, a.J);
+        }
+    });
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, kWave));
+    return x;
+}
+
+// Lanes per root of the streamed kernel for a merge of K poles at a level of
+// `merges` merges served by `groups` workgroups of W waves: enough lanes that
+// each of the groups * 64 / L root positions gets about 4 or more roots (the
+// merge's siblings assumed alike), no more than keeps 128 poles per lane per
+// pass, at most 16.
+__device__ __forceinline__ int stream_lanes(int K, int merges, int groups, int W) {
+    const long long want = (256LL * groups + (long long)K * merges - 1) / ((long long)K * merges);
+    int L = 1;
+    while (L < 16 && L < want && (long long)K >= 2LL * W * 128 * L) L <<= 1;
+    return L;
+}
+
+template <int W>
+__global__ __launch_bounds__(kWave * W) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_secular_stream(
+    int n, int depth, int kstream, const double* __restrict__ ew, const int* counts,
+    const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
+    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
+    __shared__ double part[W][6][kWave];
+    __shared__ double2 stage[W][kStageChunk];
+    __shared__ double pass_org[kWave], pass_tau[kWave];
+    __shared__ int pass_exlo[kWave], pass_nex[kWave];
+    __shared__ int pass_live;
+    const int merges = 1 << depth;
+    const int wave = uniform(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
+    double2* slot = stage[wave];
+    for (int mi = 0; mi < merges; mi++) {
+        const int merge = (int)((blockIdx.x + (unsigned)mi) % (unsigned)merges);
+        const int K = uniform(counts[kCountStride * merge]);
+        if (K < kstream) continue;
+        int s, sz;
+        node_geom(n, depth, merge, &s, &sz);
+        const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
+        const double2* __restrict__ dz = adz + s;
+        const PoleView E{dz};
+        const int q0 = (int)(((long long)K * wave) / W), q1 = (int)(((long long)K * (wave + 1)) / W);
+        const int L = uniform(stream_lanes(K, merges, (int)gridDim.x, W));
+        const int c = lane & (L - 1), lead = lane - c;  // lane c of the root at lanes lead..
+        // wave 0 owns the roots: state machine, claims, results
+        SecularSolve<PoleView> st;
+        int i = -1;  // this lane's root, -1 when it needs one
+        bool drained = false;
+        int iters = 0;
+        for (;;) {
+            if (wave == 0) {
+                if (!drained && __any(i < 0)) {
+                    // one claim per root group (its lane c = 0), numbered by rank
+                    const unsigned long long need = __ballot(i < 0 && c == 0);
+                    int base = 0;
+                    if (lane == 0)
+                        base = atomicAdd(&counts_rw[kCountStride * merge + 4], (int)__popcll(need));
+                    base = __shfl(base, 0, kWave);
+                    const int f = base + (int)__popcll(need & ((1ull << lead) - 1ull));
+                    if (i < 0 && f < K) {
+                        i = f;
+                        st.init(E, K, i, rho);
+                    }
+                    drained = __any(i < 0 && f >= K);  // the counter passed K
+                }
+                const bool active = (i >= 0);
+                pass_org[lane] = active ? st.ev_org : 0.0;
+                pass_tau[lane] = active ? st.ev_tau : 0.0;
+                pass_exlo[lane] = active ? st.ev_exlo : K;
+                pass_nex[lane] = active ? st.ev_nex : 0;
+                const bool live = __any(active);
+                if (lane == 0) pass_live = live ? 1 : 0;
+            }
+            __syncthreads();
+            if (!pass_live) break;  // the merge is done (same value in every wave)
+            const double org = pass_org[lane], tau = pass_tau[lane];
+            const int exlo = pass_exlo[lane], nex = pass_nex[lane];
+            const int wlo = uniform(wave_min(exlo)), whi = uniform(wave_max(nex > 0 ? exlo + nex : 0));
+            StreamAcc lo = {0.0, 0.0, 0.0}, hi = {0.0, 0.0, 0.0};
+            stream_side(dz, slot, q0, min(q1, wlo), c, L, org, tau, lo);
+            stream_window(dz, slot, max(q0, wlo), min(q1, whi), c, L, org, tau, exlo, nex, lo, hi);
+            stream_side(dz, slot, max(q0, whi), q1, c, L, org, tau, hi);
+            double acc[6] = {lo.S, lo.D, lo.J, hi.S, hi.D, hi.J};
+            for (int o = 1; o < L; o <<= 1) {
+#pragma unroll
+                for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+            }
+#pragma unroll
+            for (int q = 0; q < 6; q++) part[wave][q][lane] = acc[q];
+            __syncthreads();
+            // the other waves wait at the next barrier; part[] and pass_*[] are
+            // rewritten only after it
+            if (wave == 0 && i >= 0) {
+#pragma unroll
+                for (int q = 0; q < 6; q++) acc[q] = 0.0;
+                for (int w = 0; w < W; w++) {
+#pragma unroll
+                    for (int q = 0; q < 6; q++) acc[q] += part[w][q][lane];
+                }
+                SecSums sums;
+                sums.S = acc[0] + acc[3];
+                sums.A = acc[3] - acc[0];
+                sums.dpsi = acc[1];
+                sums.dphi = acc[4];
+                sums.J = acc[2] + acc[5];
+                if (st.advance(E, sums)) {
+                    const RootState& r = st.out;
+                    if (c == 0) {
+                        rlam[s + i] = r.lambda;
+                        rs[s + i] = make_double2(dz[r.org].x, r.tau);
+                        rn1[s + i] = r.near1;
+                        rn2[s + i] = r.near2;
+                        if (r.info) report_failure(info, n, s, sz);
+                        iters += r.iters;
+                    }
+                    i = -1;
+                }
+            }
+        }
+        if (wave == 0) {
+            for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
+            if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
+        }
+        __syncthreads();  // pass_*[] of this merge are read before the next merge rewrites them
     }
 }
 
@@ -909,7 +1159,7 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     const double* __restrict__ rn1, const double* __restrict__ rn2, double2* __restrict__ pw) {
     __shared__ double red[WL > 1 ? WL : 1][kGroup];
     const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
-    const int K = uniform(counts[4 * merge]);
+    const int K = uniform(counts[kCountStride * merge]);
     const int g0 = g * kGroup;
     if (K <= 2 || g0 >= K) return;
     int s, sz;
@@ -989,7 +1239,7 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
     double* __restrict__ qo2) {
     __shared__ double red[WL > 1 ? WL : 1][3][kGroup];
     const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
-    const int K = uniform(counts[4 * merge]);
+    const int K = uniform(counts[kCountStride * merge]);
     const int g0 = g * kGroup;
     if (g0 >= K) return;
     int s, sz;
@@ -1094,7 +1344,8 @@ __global__ __launch_bounds__(kThreads) void k_finalize(
     if (root && blockIdx.x == 0 && threadIdx.x == 0 && info_user) *info_user = *info;
     const int idx = part * kThreads + threadIdx.x;
     if (idx >= sz) return;
-    const int K = counts[4 * merge], G = counts[4 * merge + 1], T = counts[4 * merge + 2];
+    const int K = counts[kCountStride * merge], G = counts[kCountStride * merge + 1],
+              T = counts[kCountStride * merge + 2];
     double x, q1, q2;
     int rank;
     if (idx < K) {

@@ -8,6 +8,8 @@
 namespace brdc {
 
 constexpr int kWave = 64;
+// ints per merge record: K, G, T, secular evaluations, root-claim counter
+constexpr int kCountStride = 8;
 
 // lanes per root: about 4+ poles per lane per pass, and no more lanes than
 // the level needs to fill the machine (~kTargetLanes lanes in flight)
