@@ -118,6 +118,50 @@ int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void 
 int brdc_profile_begin(void);
 int brdc_profile_end(double *ms_out, int *launches_out, int nclasses);
 
+/*
+ * Batched solves: many independent tridiagonals of uneven sizes as one
+ * flattened merge forest.  No counterpart in the reference (which solves one
+ * matrix per call); the entry points follow the device entry's conventions.
+ *
+ * The matrices' rows are concatenated: matrix b occupies rows
+ * [off_b, off_b + n_b) with off_b = n_0 + ... + n_{b-1}; d holds the
+ * diagonals (rows entries) and e the off-diagonals in the same rows, e[off_b +
+ * i] for i < n_b - 1 (the slot of each matrix's last row is not read).
+ *
+ * A plan is built once per size list on the host (every matrix's tree and the
+ * per-height launch shapes), its tables are copied into a workspace once
+ * (brdc_batch_upload, synchronous), and brdc_batch_solve then enqueues a
+ * host-sync-free, allocation-free launch sequence -- one sweep per height of
+ * the forest -- that can be captured into a hipGraph and replayed.
+ */
+typedef struct brdc_batch_plan brdc_batch_plan;
+
+/* Builds the plan of nmat matrices of orders sizes[0..nmat) (each >= 0; all
+ * rows together below 2^29) with the options' leaf bound.  Returns a status;
+ * *plan is NULL on failure. */
+int brdc_batch_plan_create(int nmat, const int *sizes, const brdc_options *opts,
+                           brdc_batch_plan **plan);
+void brdc_batch_plan_destroy(brdc_batch_plan *plan);
+
+/* Device workspace bytes for the plan (O(rows) plus the plan tables). */
+int brdc_batch_workspace(const brdc_batch_plan *plan, size_t *bytes);
+
+/* Copies the plan tables into the workspace on stream and waits for the copy.
+ * Must precede brdc_batch_solve on that workspace (once per workspace). */
+int brdc_batch_upload(const brdc_batch_plan *plan, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+/* Enqueues the batched solve: d_dev[rows] (in: diagonals, out: each matrix's
+ * eigenvalues ascending in its own rows), e_dev[rows] (not modified),
+ * info_dev[nmat] (each matrix's INFO as brdc_dstebr defines it, rows numbered
+ * within the matrix).  Asynchronous; returns a status for the launch. */
+int brdc_batch_solve(const brdc_batch_plan *plan, double *d_dev, const double *e_dev,
+                     void *workspace, size_t workspace_bytes, void *stream, int *info_dev);
+
+/* Plan summary: up to nout of (rows, matrices, leaves, forest heights, merges,
+ * largest leaf, leaf bound).  Returns the number written or a negative status. */
+int brdc_batch_plan_stats(const brdc_batch_plan *plan, int *out, int nout);
+
 /* Human-readable name of a brdc_status. */
 const char *brdc_status_string(int status);
 

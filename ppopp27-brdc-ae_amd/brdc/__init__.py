@@ -23,8 +23,7 @@ from . import _lib
 from ._lib import BrdcError, options
 
 __all__ = ["dstebr", "dstebr_device", "workspace_bytes", "merge_stats", "Solver", "BrdcError",
-           "profile_kernels", "GraphSolve",
-           "options", "query"]
+           "profile_kernels", "GraphSolve", "BatchSolver", "options", "query"]
 
 
 def query(n: int) -> tuple[int, int, int]:
@@ -116,6 +115,76 @@ class Solver:
         for i, name in enumerate(out.dtype.names):
             out[name] = rec[:, i]
         return out
+
+
+class BatchSolver:
+    """Batched solve of many tridiagonals of uneven sizes (one flattened merge
+    forest; no counterpart in the reference, which solves one matrix a call).
+
+    ``sizes`` fixes the batch shape: the plan (every matrix's tree and the
+    per-height launch shapes) is built once and copied into this object's
+    workspace.  ``solve(d, e)`` takes the concatenated rows (see
+    :func:`brdc.generators.pack_batch`), overwrites ``d`` with each matrix's
+    eigenvalues ascending in its own rows and leaves per-matrix INFO in
+    ``self.info``; it enqueues on the current torch stream and does not
+    synchronise.
+    """
+
+    def __init__(self, sizes, device="cuda", leaf_max: int = 0):
+        import torch
+        L = _lib.load()
+        self.sizes = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1))
+        self.nmat = int(self.sizes.shape[0])
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes, dtype=np.int64)])
+        self.rows = int(self.offsets[-1])
+        self.opts = options(leaf_max)
+        self._plan = ctypes.c_void_p()
+        _lib.check(L.brdc_batch_plan_create(self.nmat, self.sizes.ctypes.data_as(_lib._ip),
+                                            ctypes.byref(self.opts), ctypes.byref(self._plan)),
+                   "batch plan")
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(L.brdc_batch_workspace(self._plan, ctypes.byref(nbytes)), "batch workspace")
+        self.bytes = int(nbytes.value)
+        self.ws = torch.empty(max(self.bytes, 256), dtype=torch.uint8, device=device)
+        if self.ws.data_ptr() % 256:
+            raise BrdcError("workspace allocation is not 256-byte aligned")
+        self.info = torch.zeros(max(self.nmat, 1), dtype=torch.int32, device=device)
+        st = torch.cuda.current_stream(self.ws.device)
+        _lib.check(L.brdc_batch_upload(self._plan, self.ws.data_ptr(), self.bytes, st.cuda_stream),
+                   "batch upload")
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan and _lib._lib is not None:
+            _lib._lib.brdc_batch_plan_destroy(plan)
+            self._plan = None
+
+    def stats(self) -> dict:
+        out = np.zeros(7, dtype=np.int32)
+        k = _lib.load().brdc_batch_plan_stats(self._plan, out.ctypes.data_as(_lib._ip), 7)
+        if k < 0:
+            _lib.check(-k, "batch plan stats")
+        keys = ("rows", "matrices", "leaves", "heights", "merges", "largest_leaf", "leaf_bound")
+        return {kk: int(v) for kk, v in zip(keys, out[:k])}
+
+    def solve(self, d, e, stream=None):
+        import torch
+        if d.dtype != torch.float64 or e.dtype != torch.float64:
+            raise TypeError("d and e must be float64")
+        if not (d.is_cuda and e.is_cuda and d.is_contiguous() and e.is_contiguous()):
+            raise ValueError("d and e must be contiguous device tensors")
+        if d.numel() != self.rows or e.numel() < self.rows:
+            raise ValueError("d and e must hold the batch's concatenated rows")
+        st = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(_lib.load().brdc_batch_solve(self._plan, d.data_ptr(), e.data_ptr(),
+                                                self.ws.data_ptr(), self.bytes, st.cuda_stream,
+                                                self.info.data_ptr()),
+                   "brdc_batch_solve")
+        return d
+
+    def split(self, w):
+        """Per-matrix views of concatenated rows (numpy or torch)."""
+        return [w[int(self.offsets[b]):int(self.offsets[b + 1])] for b in range(self.nmat)]
 
 
 def dstebr_device(d, e, leaf_max: int = 0):

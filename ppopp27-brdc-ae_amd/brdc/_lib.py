@@ -19,7 +19,9 @@ STATUS = {0: "success", 2: "invalid pointer", 3: "invalid value", 4: "invalid wo
 #: every symbol include/brdc.h declares
 EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
            "brdc_dstebr_device_workspace_ex", "brdc_dstebr_device", "brdc_dstebr_device_ex",
-           "brdc_merge_stats", "brdc_status_string", "brdc_profile_begin", "brdc_profile_end")
+           "brdc_merge_stats", "brdc_status_string", "brdc_profile_begin", "brdc_profile_end",
+           "brdc_batch_plan_create", "brdc_batch_plan_destroy", "brdc_batch_workspace",
+           "brdc_batch_upload", "brdc_batch_solve", "brdc_batch_plan_stats")
 
 KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
                   "finalize")
@@ -71,6 +73,19 @@ def load() -> ctypes.CDLL:
     L.brdc_profile_end.argtypes = [_dp, _ip, ctypes.c_int]
     L.brdc_status_string.restype = ctypes.c_char_p
     L.brdc_status_string.argtypes = [ctypes.c_int]
+    L.brdc_batch_plan_create.restype = ctypes.c_int
+    L.brdc_batch_plan_create.argtypes = [ctypes.c_int, _ip, ctypes.POINTER(Options),
+                                         ctypes.POINTER(ctypes.c_void_p)]
+    L.brdc_batch_plan_destroy.restype = None
+    L.brdc_batch_plan_destroy.argtypes = [_vp]
+    L.brdc_batch_workspace.restype = ctypes.c_int
+    L.brdc_batch_workspace.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.brdc_batch_upload.restype = ctypes.c_int
+    L.brdc_batch_upload.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
+    L.brdc_batch_solve.restype = ctypes.c_int
+    L.brdc_batch_solve.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp]
+    L.brdc_batch_plan_stats.restype = ctypes.c_int
+    L.brdc_batch_plan_stats.argtypes = [_vp, _ip, ctypes.c_int]
     _lib = L
     return L
 

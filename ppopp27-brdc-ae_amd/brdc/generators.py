@@ -97,6 +97,32 @@ def batched_sizes(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
     return np.clip(s, lo, hi)
 
 
+def batched_workload(count: int = 2048, lo: int = 32, hi: int = 4096, config: int = 4,
+                     kind: str = "normal"):
+    """The batched workload: ``count`` sizes log-uniform in [lo, hi] (sizes
+    seed ``seed_for(config, 0)``), matrix b drawn by ``generate(kind, n_b,
+    seed_for(config, n_b, b))``.  Returns (sizes, [(D, E), ...])."""
+    sizes = batched_sizes(count, lo, hi, seed_for(config, 0))
+    mats = [generate(kind, int(nb), seed_for(config, int(nb), b)) for b, nb in enumerate(sizes)]
+    return sizes, mats
+
+
+def pack_batch(mats) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Concatenate matrices for the batched entry: (D rows, E rows, offsets).
+    Matrix b owns rows [off_b, off_b+n_b); its E sits in the first n_b - 1
+    of them and the last row's E slot is 0."""
+    sizes = np.array([d.shape[0] for d, _ in mats], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    rows = int(off[-1])
+    d_cat = np.zeros(max(rows, 1))
+    e_cat = np.zeros(max(rows, 1))
+    for b, (d, e) in enumerate(mats):
+        o, nb = int(off[b]), int(sizes[b])
+        d_cat[o:o + nb] = d
+        e_cat[o:o + max(nb - 1, 0)] = e[:max(nb - 1, 0)]
+    return d_cat[:rows], e_cat[:rows], off
+
+
 def write_de_bin(path: str, d: np.ndarray, e: np.ndarray) -> None:
     n = int(d.shape[0])
     if e.shape[0] != max(n - 1, 0):

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <climits>
+#include <new>
 #include <vector>
 
 #include "../../include/brdc.h"
@@ -50,7 +51,9 @@ constexpr size_t kAlign = 256;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// Workspace carve-up; identical for the query and the run.
+// Workspace carve-up; identical for the query and the run.  Per-row slots
+// cover `rows` rows (one matrix, or a batch's concatenated rows); the
+// per-matrix, plan and count records follow.
 struct Layout {
     size_t scal, ew;
     size_t sd[2], slo[2], shi[2];
@@ -62,24 +65,21 @@ struct Layout {
     size_t pw, qo1, qo2;
     size_t P, rend, fl;
     size_t counts, info;
+    size_t plan, moff;  // batch only
     size_t total;
-    int levels;
-    int merges;  // 2^levels (merge records, index (2^k - 1) + i at depth k)
 };
 
-Layout make_layout(int n, int leaf) {
+Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items) {
     Layout L{};
-    const size_t nd = (size_t)std::max(n, 1) * sizeof(double);
-    const size_t ni = (size_t)std::max(n, 1) * sizeof(int);
+    const size_t nd = (size_t)std::max(rows, 1) * sizeof(double);
+    const size_t ni = (size_t)std::max(rows, 1) * sizeof(int);
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t at = off;
         off = align_up(off + bytes);
         return at;
     };
-    L.levels = (n > 0) ? tree_levels(n, leaf) : 0;
-    L.merges = 1 << L.levels;
-    L.scal = take(4 * sizeof(double));
+    L.scal = take(2 * (size_t)std::max(nmat, 1) * sizeof(double));
     L.ew = take(nd);
     for (int b = 0; b < 2; b++) {
         L.sd[b] = take(nd);
@@ -110,10 +110,18 @@ Layout make_layout(int n, int leaf) {
     L.P = take(ni);
     L.rend = take(ni);
     L.fl = take(ni);
-    L.counts = take((size_t)kCountStride * L.merges * sizeof(int));
+    L.counts = take((size_t)kCountStride * std::max<size_t>(merge_records, 1) * sizeof(int));
     L.info = take(sizeof(int));
+    L.plan = plan_items ? take(plan_items * sizeof(int4)) : 0;
+    L.moff = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
     L.total = off;
     return L;
+}
+
+// One matrix of order n: merge records at index (2^depth - 1) + i.
+Layout tree_layout(int n, int leaf) {
+    const int levels = (n > 0) ? tree_levels(n, leaf) : 0;
+    return make_layout(n, 1, (size_t)1 << levels, 0);
 }
 
 int leaf_of(const brdc_options* o) {
@@ -189,41 +197,81 @@ void launch_deflate(int maxsz, int merges, hipStream_t st, Args... args) {
         hipLaunchKernelGGL(k_deflate<1024>, dim3(merges), dim3(1024), 0, st, args...);
 }
 
-int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, hipStream_t st,
-                  int* info_user, int leaf) {
+// One merge level of a launch sequence: all merges of one tree depth (one
+// matrix) or of one height above the leaves (a batch's merge forest).
+struct LevelRun {
+    Geo geo;
+    int merges;
+    int maxsz;
+    size_t counts_rec;  // first merge record of the level
+    bool any_nonroot;   // some merge keeps boundary rows (wprod/rows run)
+};
+
+// A whole solve: rows, matrices, leaves and levels (deepest first).
+struct Schedule {
+    int rows = 0, nmat = 1;
+    const int* moff = nullptr;  // device: first row of each matrix (batch), else nullptr
+    Geo leaves{};
+    int nleaves = 0, leaf_rows = 0;
+    std::vector<LevelRun> levels;
+};
+
+Schedule tree_schedule(int n, int leaf) {
+    Schedule S;
+    S.rows = n;
+    const int levels = tree_levels(n, leaf);
+    S.leaves = Geo{n, levels, nullptr};
+    S.nleaves = 1 << levels;
+    S.leaf_rows = max_node_size(n, levels);
+    for (int depth = levels - 1; depth >= 0; depth--) {
+        LevelRun r;
+        r.geo = Geo{n, depth, nullptr};
+        r.merges = 1 << depth;
+        r.maxsz = max_node_size(n, depth);
+        r.counts_rec = (size_t)r.merges - 1;
+        r.any_nonroot = (depth > 0);
+        S.levels.push_back(r);
+    }
+    return S;
+}
+
+// Enqueues a solve on st.  info: the INFO word(s) written by the kernels (one
+// matrix: the workspace word, copied to info_user by the last kernel; batch:
+// the caller's per-matrix array).
+int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layout& L,
+            hipStream_t st, int* info, int* info_user) {
+    const int n = S.rows;
     double* scal = at<double>(ws, L.scal);
     double* ew = at<double>(ws, L.ew);
-    int* info = at<int>(ws, L.info);
     int* counts_all = at<int>(ws, L.counts);
 
     {
         ProfScope ps(kNorm, st);
-        hipLaunchKernelGGL(k_norm, dim3(1), dim3(1024), 0, st, n, d, e, scal, info, info_user);
+        hipLaunchKernelGGL(k_norm, dim3(S.nmat), dim3(1024), 0, st, n, S.moff, d, e, scal, info,
+                           info_user);
     }
     BRDC_LAUNCH_CHECK();
     if (n <= 1) return BRDC_STATUS_SUCCESS;
     {
         ProfScope ps(kPrepE, st);
-        hipLaunchKernelGGL(k_prep_e, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, d, e, scal,
-                           ew);
+        hipLaunchKernelGGL(k_prep_e, dim3((n + 255) / 256), dim3(256), 0, st, n, S.moff, S.nmat,
+                           d, e, scal, ew);
     }
     BRDC_LAUNCH_CHECK();
 
-    const int levels = L.levels;
     int cur = 0;
     double* sd[2] = {at<double>(ws, L.sd[0]), at<double>(ws, L.sd[1])};
     double* slo[2] = {at<double>(ws, L.slo[0]), at<double>(ws, L.slo[1])};
     double* shi[2] = {at<double>(ws, L.shi[0]), at<double>(ws, L.shi[1])};
-    const int single = (levels == 0);
     {
         ProfScope ps(kLeaf, st);
-        const int leaf_rows = max_node_size(n, levels);
-        auto kern = (leaf_rows <= 16) ? k_leaf<16> : (leaf_rows <= 32) ? k_leaf<32> : k_leaf<64>;
-        hipLaunchKernelGGL(kern, dim3(1 << levels), dim3(kWave), 0, st, n, levels, d, ew, scal,
-                           single ? d : sd[cur], slo[cur], shi[cur], info, single, info_user);
+        auto kern = (S.leaf_rows <= 16) ? k_leaf<16> : (S.leaf_rows <= 32) ? k_leaf<32> : k_leaf<64>;
+        hipLaunchKernelGGL(kern, dim3(S.nleaves), dim3(kWave), 0, st, S.leaves, S.moff,
+                           (const double*)d, (const double*)ew, (const double*)scal, d, sd[cur],
+                           slo[cur], shi[cur], info, S.levels.empty() ? info_user : nullptr);
     }
     BRDC_LAUNCH_CHECK();
-    if (single) return BRDC_STATUS_SUCCESS;
+    if (S.levels.empty()) return BRDC_STATUS_SUCCESS;
 
     double* md = at<double>(ws, L.md);
     double* mz = at<double>(ws, L.mz);
@@ -250,28 +298,28 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
     int* rend = at<int>(ws, L.rend);
     int* fl = at<int>(ws, L.fl);
 
-    for (int depth = levels - 1; depth >= 0; depth--) {
-        const int merges = 1 << depth;
-        const int maxsz = max_node_size(n, depth);
+    for (const LevelRun& lv : S.levels) {
+        const Geo g = lv.geo;
+        const int merges = lv.merges;
+        const int maxsz = lv.maxsz;
         const int bpm = (maxsz + kThreads - 1) / kThreads;
         const dim3 grid(merges * bpm);
         // the O(K^2) kernels: one block per 64 roots/poles, WL waves wide
         const int bpg = (maxsz + kGroup - 1) / kGroup;
         const int WL = waves_for(maxsz, kMaxWaves);
-        int* counts = counts_all + kCountStride * (merges - 1);
+        int* counts = counts_all + kCountStride * lv.counts_rec;
         const int nxt = cur ^ 1;
-        const int root = (depth == 0);
 
         {
             ProfScope ps(kPrepare, st);
-            hipLaunchKernelGGL(k_merge_prepare, grid, dim3(kThreads), 0, st, n, depth, bpm,
-                               sd[cur], slo[cur], shi[cur], ew, md, mz, mq1, mq2);
+            hipLaunchKernelGGL(k_merge_prepare, grid, dim3(kThreads), 0, st, g, bpm, sd[cur],
+                               slo[cur], shi[cur], ew, md, mz, mq1, mq2);
         }
         BRDC_LAUNCH_CHECK();
         {
             ProfScope ps(kDeflate, st);
-            launch_deflate(maxsz, merges, st, n, depth, ew, md, mz, mq1, mq2, P, rend, fl, ad, az,
-                           adz, aq, gd, gq1, gq2, td, tq1, tq2, counts);
+            launch_deflate(maxsz, merges, st, g, ew, md, mz, mq1, mq2, P, rend, fl, ad, az, adz,
+                           aq, gd, gq1, gq2, td, tq1, tq2, counts);
         }
         BRDC_LAUNCH_CHECK();
         // merges of K >= kStreamMinK poles go to the streamed kernel, the
@@ -285,49 +333,102 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
             const int waves = std::min(merges * slots, kSecMaxWaves);
             const int blocks = (waves + kSecWaves - 1) / kSecWaves;
             hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves), 0, st,
-                               n, depth, slots, streamed ? kStreamMinK : INT_MAX,
+                               g, merges, slots, streamed ? kStreamMinK : INT_MAX,
                                (const double*)ew, (const int*)counts, (const double*)ad,
                                (const double*)az, (const double2*)adz, rlam, rs, rn1, rn2, counts,
-                               info);
+                               S.moff, info);
         }
         BRDC_LAUNCH_CHECK();
         if (streamed) {
             ProfScope ps(kSecular, st);
             const int groups = std::min(stream_groups(), (n + kWave - 1) / kWave);
             hipLaunchKernelGGL(k_secular_stream<kStreamWaves>, dim3(groups),
-                               dim3(kWave * kStreamWaves), 0, st, n, depth, kStreamMinK,
+                               dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
                                (const double*)ew, (const int*)counts, (const double2*)adz, rlam,
-                               rs, rn1, rn2, counts, info);
+                               rs, rn1, rn2, counts, S.moff, info);
             BRDC_LAUNCH_CHECK();
         }
-        if (!root) {
+        if (lv.any_nonroot) {
             {
                 ProfScope ps(kWprod, st);
                 launch_wide(k_wprod<1>, k_wprod<2>, k_wprod<4>, k_wprod<8>, WL,
-                            merges * bpg, st, n, depth, bpg, (const int*)counts,
-                            (const double*)ad, (const double*)az, (const double2*)rs,
-                            (const double*)rn1, (const double*)rn2, pw);
+                            merges * bpg, st, g, bpg, (const int*)counts, (const double*)ad,
+                            (const double*)az, (const double2*)rs, (const double*)rn1,
+                            (const double*)rn2, pw);
             }
             BRDC_LAUNCH_CHECK();
             {
                 ProfScope ps(kRows, st);
                 launch_wide(k_rows<1>, k_rows<2>, k_rows<4>, k_rows<8>, WL,
-                            merges * bpg, st, n, depth, bpg, (const int*)counts,
-                            (const double2*)aq, (const double2*)pw, (const double2*)rs,
-                            (const double*)rn1, (const double*)rn2, qo1, qo2);
+                            merges * bpg, st, g, bpg, (const int*)counts, (const double2*)aq,
+                            (const double2*)pw, (const double2*)rs, (const double*)rn1,
+                            (const double*)rn2, qo1, qo2);
             }
             BRDC_LAUNCH_CHECK();
         }
         {
             ProfScope ps(kFinalize, st);
-            hipLaunchKernelGGL(k_finalize, grid, dim3(kThreads), 0, st, n, depth, bpm, counts,
-                               rlam, qo1, qo2, gd, gq1, gq2, td, tq1, tq2, root ? d : sd[nxt],
-                               slo[nxt], shi[nxt], root, scal, info, info_user);
+            hipLaunchKernelGGL(k_finalize, grid, dim3(kThreads), 0, st, g, bpm, counts, rlam, qo1,
+                               qo2, gd, gq1, gq2, td, tq1, tq2, d, sd[nxt], slo[nxt], shi[nxt],
+                               (const double*)scal, (const int*)info, info_user);
         }
         BRDC_LAUNCH_CHECK();
         cur = nxt;
     }
     return BRDC_STATUS_SUCCESS;
+}
+
+int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, hipStream_t st,
+                  int* info_user, int leaf) {
+    return enqueue(tree_schedule(n, leaf), d, e, ws, L, st, at<int>(ws, L.info), info_user);
+}
+
+// ---------------------------------------------------------------------------
+// Batch plan: the flattened merge forest of a batch of matrices.  Each
+// matrix has its own tree (same leaf bound); level h of the forest holds
+// every matrix's merges h levels above its leaves, so one launch sequence
+// per height serves all matrices.  Built once on the host per size list.
+struct HostLevel {
+    int merges = 0, maxsz = 0;
+    size_t plan_at = 0, counts_rec = 0;
+    bool any_nonroot = false;
+};
+
+}  // namespace
+}  // namespace brdc
+
+struct brdc_batch_plan {
+    int nmat = 0, rows = 0, leaf = 0;
+    std::vector<int> moff;           // nmat + 1
+    std::vector<int4> items;         // leaves, then every level's merges
+    int nleaves = 0, leaf_rows = 0;
+    std::vector<brdc::HostLevel> levels;  // height 1 upward
+    size_t merge_records = 0;
+    brdc::Layout layout;
+};
+
+namespace brdc {
+namespace {
+
+Schedule batch_schedule(const brdc_batch_plan& p, void* ws) {
+    Schedule S;
+    S.rows = p.rows;
+    S.nmat = p.nmat;
+    S.moff = at<int>(ws, p.layout.moff);
+    const int4* plan = at<int4>(ws, p.layout.plan);
+    S.leaves = Geo{p.rows, -1, plan};
+    S.nleaves = p.nleaves;
+    S.leaf_rows = p.leaf_rows;
+    for (const HostLevel& h : p.levels) {
+        LevelRun r;
+        r.geo = Geo{p.rows, -1, plan + h.plan_at};
+        r.merges = h.merges;
+        r.maxsz = h.maxsz;
+        r.counts_rec = h.counts_rec;
+        r.any_nonroot = h.any_nonroot;
+        S.levels.push_back(r);
+    }
+    return S;
 }
 
 }  // namespace
@@ -353,7 +454,7 @@ const char* brdc_status_string(int status) {
 int brdc_dstebr_device_workspace_ex(int n, const brdc_options* opts, size_t* bytes) {
     if (!bytes) return BRDC_STATUS_INVALID_POINTER;
     if (n < 0 || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
-    *bytes = make_layout(n, leaf_of(opts)).total;
+    *bytes = tree_layout(n, leaf_of(opts)).total;
     return BRDC_STATUS_SUCCESS;
 }
 
@@ -368,7 +469,7 @@ int brdc_dstebr_device_ex(int n, double* d_dev, const double* e_dev, void* works
     if (n == 0) return BRDC_STATUS_SUCCESS;
     if (!d_dev || (n > 1 && !e_dev) || !workspace) return BRDC_STATUS_INVALID_POINTER;
     const int leaf = leaf_of(opts);
-    const Layout L = make_layout(n, leaf);
+    const Layout L = tree_layout(n, leaf);
     if (workspace_bytes < L.total || (reinterpret_cast<uintptr_t>(workspace) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
     return enqueue_solve(n, d_dev, e_dev, workspace, L, static_cast<hipStream_t>(stream), info_dev,
@@ -385,9 +486,11 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
                      const brdc_options* opts, int* out_host, int max_merges) {
     if (n < 0 || !options_ok(opts)) return -BRDC_STATUS_INVALID_VALUE;
     if (!workspace || (!out_host && max_merges > 0)) return -BRDC_STATUS_INVALID_POINTER;
-    const Layout L = make_layout(n, leaf_of(opts));
+    const int leaf = leaf_of(opts);
+    const Layout L = tree_layout(n, leaf);
     if (workspace_bytes < L.total) return -BRDC_STATUS_INVALID_WORKSPACE;
-    const int total = (n > 1) ? (1 << L.levels) - 1 : 0;
+    const int levels = (n > 0) ? tree_levels(n, leaf) : 0;
+    const int total = (n > 1) ? (1 << levels) - 1 : 0;
     if (total == 0) return 0;
     std::vector<int> counts((size_t)kCountStride * total);
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -397,7 +500,7 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
         hipStreamSynchronize(st) != hipSuccess)
         return -BRDC_STATUS_HIP_ERROR;
     int rec = 0;
-    for (int depth = 0; depth < L.levels; depth++) {
+    for (int depth = 0; depth < levels; depth++) {
         for (int i = 0; i < (1 << depth); i++) {
             if (rec >= max_merges) return total;
             int s, sz;
@@ -466,7 +569,7 @@ int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork
     }
     if (n <= 1) return 0;
 
-    const Layout L = make_layout(n, kDefaultLeaf);
+    const Layout L = tree_layout(n, kDefaultLeaf);
     double *dd = nullptr, *de = nullptr;
     void* ws = nullptr;
     int* dinfo = nullptr;
@@ -501,6 +604,116 @@ int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork
 void brdc_dstebr_(const int* n, double* d, double* e, double* work, const int* lwork, int* iwork,
                   const int* liwork, int* info) {
     *info = brdc_dstebr(*n, d, e, work, *lwork, iwork, *liwork);
+}
+
+int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
+                           brdc_batch_plan** plan) {
+    if (!plan) return BRDC_STATUS_INVALID_POINTER;
+    *plan = nullptr;
+    if (nmat > 0 && !sizes) return BRDC_STATUS_INVALID_POINTER;
+    if (nmat < 0 || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
+    long long rows = 0;
+    for (int b = 0; b < nmat; b++) {
+        if (sizes[b] < 0) return BRDC_STATUS_INVALID_VALUE;
+        rows += sizes[b];
+    }
+    // row indices and plan entries are int32; per-row pair slots hold 2*rows doubles
+    if (rows > (long long)(INT_MAX / 4)) return BRDC_STATUS_INVALID_VALUE;
+    brdc_batch_plan* p = new (std::nothrow) brdc_batch_plan;
+    if (!p) return BRDC_STATUS_ALLOC_FAILED;
+    p->nmat = nmat;
+    p->rows = (int)rows;
+    p->leaf = leaf_of(opts);
+    p->moff.resize((size_t)nmat + 1);
+    std::vector<int> lev((size_t)nmat, 0);
+    int H = 0;
+    p->moff[0] = 0;
+    for (int b = 0; b < nmat; b++) {
+        p->moff[b + 1] = p->moff[b] + sizes[b];
+        lev[b] = (sizes[b] > 0) ? tree_levels(sizes[b], p->leaf) : -1;
+        H = std::max(H, lev[b]);
+    }
+    // leaves of every matrix (a matrix that fits one leaf is solved there)
+    for (int b = 0; b < nmat; b++) {
+        if (lev[b] < 0) continue;
+        for (int i = 0; i < (1 << lev[b]); i++) {
+            int s, sz;
+            node_geom(sizes[b], lev[b], i, &s, &sz);
+            p->items.push_back(make_int4(p->moff[b] + s, sz, b, lev[b] == 0 ? kItemRoot : 0));
+            p->leaf_rows = std::max(p->leaf_rows, sz);
+        }
+    }
+    p->nleaves = (int)p->items.size();
+    // merges h levels above the leaves, h = 1 .. H
+    for (int h = 1; h <= H; h++) {
+        HostLevel L;
+        L.plan_at = p->items.size();
+        L.counts_rec = p->merge_records;
+        for (int b = 0; b < nmat; b++) {
+            if (lev[b] < h) continue;
+            const int depth = lev[b] - h;
+            for (int i = 0; i < (1 << depth); i++) {
+                int s, sz;
+                node_geom(sizes[b], depth, i, &s, &sz);
+                p->items.push_back(make_int4(p->moff[b] + s, sz, b, depth == 0 ? kItemRoot : 0));
+                L.maxsz = std::max(L.maxsz, sz);
+                L.any_nonroot = L.any_nonroot || depth > 0;
+            }
+        }
+        L.merges = (int)(p->items.size() - L.plan_at);
+        p->merge_records += (size_t)L.merges;
+        p->levels.push_back(L);
+    }
+    p->layout = make_layout(p->rows, nmat, p->merge_records, std::max<size_t>(p->items.size(), 1));
+    *plan = p;
+    return BRDC_STATUS_SUCCESS;
+}
+
+void brdc_batch_plan_destroy(brdc_batch_plan* plan) { delete plan; }
+
+int brdc_batch_workspace(const brdc_batch_plan* plan, size_t* bytes) {
+    if (!plan || !bytes) return BRDC_STATUS_INVALID_POINTER;
+    *bytes = plan->layout.total;
+    return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_batch_plan_stats(const brdc_batch_plan* plan, int* out, int nout) {
+    if (!plan || (!out && nout > 0)) return -BRDC_STATUS_INVALID_POINTER;
+    const int v[7] = {plan->rows,       plan->nmat, plan->nleaves, (int)plan->levels.size(),
+                      (int)plan->merge_records, plan->leaf_rows, plan->leaf};
+    const int k = std::min(nout, 7);
+    for (int i = 0; i < k; i++) out[i] = v[i];
+    return k;
+}
+
+int brdc_batch_upload(const brdc_batch_plan* plan, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+    if (!plan || !workspace) return BRDC_STATUS_INVALID_POINTER;
+    const Layout& L = plan->layout;
+    if (workspace_bytes < L.total || (reinterpret_cast<uintptr_t>(workspace) % kAlign) != 0)
+        return BRDC_STATUS_INVALID_WORKSPACE;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!plan->items.empty() &&
+        hipMemcpyAsync(at<int4>(workspace, L.plan), plan->items.data(),
+                       plan->items.size() * sizeof(int4), hipMemcpyHostToDevice, st) != hipSuccess)
+        return BRDC_STATUS_HIP_ERROR;
+    if (hipMemcpyAsync(at<int>(workspace, L.moff), plan->moff.data(),
+                       plan->moff.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return BRDC_STATUS_HIP_ERROR;
+    return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_batch_solve(const brdc_batch_plan* plan, double* d_dev, const double* e_dev,
+                     void* workspace, size_t workspace_bytes, void* stream, int* info_dev) {
+    if (!plan) return BRDC_STATUS_INVALID_POINTER;
+    if (plan->nmat == 0) return BRDC_STATUS_SUCCESS;
+    if (!d_dev || !e_dev || !workspace || !info_dev) return BRDC_STATUS_INVALID_POINTER;
+    const Layout& L = plan->layout;
+    if (workspace_bytes < L.total || (reinterpret_cast<uintptr_t>(workspace) % kAlign) != 0)
+        return BRDC_STATUS_INVALID_WORKSPACE;
+    return enqueue(batch_schedule(*plan, workspace), d_dev, e_dev, workspace, L,
+                   static_cast<hipStream_t>(stream), info_dev, nullptr);
 }
 
 }  // extern "C"

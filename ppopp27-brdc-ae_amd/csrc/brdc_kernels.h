@@ -103,19 +103,73 @@ __device__ __forceinline__ void report_failure(int* info, int n, int start, int 
 }
 
 // ---------------------------------------------------------------------------
+// Geometry of the items (leaves or merges) of one launch.  A single matrix
+// derives node (depth, idx) from the bits of idx (plan == nullptr); a batch
+// of matrices -- one flattened merge forest over the concatenated rows --
+// reads each item from a plan entry (first row, size, matrix, flags).
+struct Item {
+    int s, sz, mat, flags;
+};
+constexpr int kItemRoot = 1;  // merge: its matrix's root merge; leaf: the whole matrix
+
+struct Geo {
+    int n;            // rows (all matrices' rows for a batch)
+    int depth;        // tree depth of the launch's items (single matrix)
+    const int4* plan;  // batch: one entry per item, else nullptr
+    __device__ __forceinline__ Item item(int idx) const {
+        Item it;
+        if (plan) {
+            const int4 p = plan[idx];
+            it.s = p.x;
+            it.sz = p.y;
+            it.mat = p.z;
+            it.flags = p.w;
+        } else {
+            node_geom(n, depth, idx, &it.s, &it.sz);
+            it.mat = 0;
+            it.flags = (depth == 0) ? kItemRoot : 0;
+        }
+        return it;
+    }
+};
+
+// Where an item reports a failure: its matrix's INFO word, in the matrix's
+// own row numbering (moff: first row of each matrix, nullptr for one matrix).
+struct FailSite {
+    int* info;
+    int n, s;
+};
+__device__ __forceinline__ FailSite fail_site(int* info, const int* moff, int n, const Item& it) {
+    if (!moff) return FailSite{info, n, it.s};
+    const int o = moff[it.mat];
+    return FailSite{info + it.mat, moff[it.mat + 1] - o, it.s - o};
+}
+__device__ __forceinline__ void report_item_failure(const FailSite& f, int size) {
+    report_failure(f.info, f.n, f.s, size);
+}
+
+// ---------------------------------------------------------------------------
 // max-norm, finiteness and scale (DLANST('M') + DLASCL, dstedc_dc_br.f:86-118)
-__global__ __launch_bounds__(1024) void k_norm(int n, const double* __restrict__ d,
+// One block per matrix (moff: first row of each matrix; nullptr: one matrix
+// of n rows).
+__global__ __launch_bounds__(1024) void k_norm(int n, const int* __restrict__ moff,
+                                               const double* __restrict__ d,
                                                const double* __restrict__ e, double* scal,
                                                int* info, int* info_user) {
     __shared__ double red[16];
     __shared__ int bad[16];
+    const int b = blockIdx.x;
+    const int off = moff ? moff[b] : 0;
+    const int nb = moff ? moff[b + 1] - off : n;
+    d += off;
+    e += off;
     double m = 0.0;
     int nonfinite = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         const double x = d[i];
         nonfinite |= !isfinite(x);
         m = fmax(m, fabs(x));
-        if (i < n - 1) {
+        if (i < nb - 1) {
             const double y = e[i];
             nonfinite |= !isfinite(y);
             m = fmax(m, fabs(y));
@@ -139,23 +193,41 @@ __global__ __launch_bounds__(1024) void k_norm(int n, const double* __restrict__
             bb |= bad[w];
         }
         const bool ok = mm > 0.0;
-        scal[0] = ok ? 1.0 / mm : 1.0;  // scale into [-1, 1]
-        scal[1] = ok ? mm : 1.0;        // undo
-        *info = bb ? n + 1 : 0;
-        if (info_user && n <= 1) *info_user = *info;
+        scal[2 * b] = ok ? 1.0 / mm : 1.0;  // scale into [-1, 1]
+        scal[2 * b + 1] = ok ? mm : 1.0;    // undo
+        info[moff ? b : 0] = bb ? nb + 1 : 0;
+        if (info_user && nb <= 1) *info_user = info[0];
     }
 }
 
 // scaled off-diagonal with negligible entries zeroed (dstedc_dc_br.f:94-105):
 // a zero there splits the matrix, which the tree then handles as a rho = 0
 // merge (everything deflates) or inside a leaf
-__global__ void k_prep_e(int n, const double* __restrict__ d, const double* __restrict__ e,
+// Every row gets an entry; the last row of each matrix gets 0, so a batch's
+// matrices are decoupled wherever a leaf or merge reads across their border.
+__global__ void k_prep_e(int n, const int* __restrict__ moff, int nmat,
+                         const double* __restrict__ d, const double* __restrict__ e,
                          const double* __restrict__ scal, double* ew) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
+    if (i >= n) return;
+    int b = 0, last = n - 1;
+    if (moff) {  // the matrix holding row i: moff[b] <= i < moff[b + 1]
+        int lo = 0, hi = nmat;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (moff[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        b = lo;
+        last = moff[b + 1] - 1;
+    }
+    if (i == last) {
+        ew[i] = 0.0;
+        return;
+    }
     const double tiny = kEps * sqrt(fabs(d[i])) * sqrt(fabs(d[i + 1]));
     const double x = e[i];
-    ew[i] = (fabs(x) > tiny) ? x * scal[0] : 0.0;
+    ew[i] = (fabs(x) > tiny) ? x * scal[2 * b] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -169,21 +241,23 @@ __global__ void k_prep_e(int n, const double* __restrict__ d, const double* __re
 // dlaed0_br.f:354-371).  The rank-one cut corrections (dlaed0_br.f:95-103)
 // are applied to the leaf's end rows on load.
 template <int MAXM>
-__global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* __restrict__ d_in,
+__global__ __launch_bounds__(kWave) void k_leaf(Geo g, const int* __restrict__ moff,
+                                                const double* __restrict__ d_in,
                                                 const double* __restrict__ ew,
-                                                const double* __restrict__ scal, double* out_d,
-                                                double* out_lo, double* out_hi, int* info,
-                                                int single, int* info_user) {
+                                                const double* __restrict__ scal, double* out_final,
+                                                double* out_d, double* out_lo, double* out_hi,
+                                                int* info, int* info_user) {
     constexpr int LD = MAXM + 1;
     __shared__ double A[MAXM * LD];
     __shared__ double V0[MAXM], V1[MAXM];
     __shared__ double RC[MAXM / 2], RS[MAXM / 2];
     __shared__ int RP[MAXM / 2], RQ[MAXM / 2];
     __shared__ int changed;
-    int s, m;
-    node_geom(n, depth, blockIdx.x, &s, &m);
+    const Item it = g.item(blockIdx.x);
+    const int s = it.s, m = it.sz, n = g.n;
+    const bool single = (it.flags & kItemRoot) != 0;  // the leaf is its whole matrix
     const int lane = threadIdx.x;
-    const double scale = scal[0];
+    const double scale = scal[2 * it.mat];
     for (int idx = lane; idx < MAXM * LD; idx += kWave) A[idx] = 0.0;
     __syncthreads();
     if (lane < m) {
@@ -278,7 +352,7 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
         converged = (changed == 0);
         __syncthreads();
     }
-    if (!converged && lane == 0) report_failure(info, n, s, m);
+    if (!converged && lane == 0) report_item_failure(fail_site(info, moff, n, it), m);
     // sort ascending, carrying the two rows
     __shared__ double L[MAXM];
     if (lane < m) L[lane] = A[lane * LD + lane];
@@ -291,14 +365,14 @@ __global__ __launch_bounds__(kWave) void k_leaf(int n, int depth, const double* 
             rank += (y < x) || (y == x && k < lane);
         }
         if (single) {
-            out_d[rank] = x * scal[1];
+            out_final[s + rank] = x * scal[2 * it.mat + 1];
         } else {
             out_d[s + rank] = x;
             out_lo[s + rank] = V0[lane];
             out_hi[s + rank] = V1[lane];
         }
     }
-    if (single && lane == 0 && info_user) *info_user = *info;
+    if (single && lane == 0 && info_user) *info_user = *info;  // one matrix only
 }
 
 // ---------------------------------------------------------------------------
@@ -328,12 +402,12 @@ __device__ __forceinline__ int count_less_equal(const double* a, int len, double
 }
 
 __global__ __launch_bounds__(kThreads) void k_merge_prepare(
-    int n, int depth, int bpm, const double* __restrict__ sd, const double* __restrict__ slo,
+    Geo g, int bpm, const double* __restrict__ sd, const double* __restrict__ slo,
     const double* __restrict__ shi, const double* __restrict__ ew, double* __restrict__ md,
     double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2) {
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    int s, sz;
-    node_geom(n, depth, merge, &s, &sz);
+    const Item it = g.item(merge);
+    const int s = it.s, sz = it.sz;
     const int idx = part * kThreads + threadIdx.x;
     if (idx >= sz) return;
     const int cut = sz / 2;
@@ -520,7 +594,7 @@ __device__ __forceinline__ int speculative_end(int s, int t, int m, int cap, dou
 // compaction; cross-wave offsets come from per-wave totals in LDS.
 template <int T>
 __global__ __launch_bounds__(T) void k_deflate(
-    int n, int depth, const double* __restrict__ ew, double* __restrict__ md,
+    Geo g, const double* __restrict__ ew, double* __restrict__ md,
     double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2, int* __restrict__ P,
     int* __restrict__ rend, int* __restrict__ fl, double* __restrict__ ad, double* __restrict__ az,
     double2* __restrict__ adz, double2* __restrict__ aq, double* __restrict__ gd,
@@ -530,8 +604,8 @@ __global__ __launch_bounds__(T) void k_deflate(
     __shared__ int wa[NW], wb[NW];
     __shared__ int s_fix;
     const int merge = blockIdx.x;
-    int s, sz;
-    node_geom(n, depth, merge, &s, &sz);
+    const Item it = g.item(merge);
+    const int s = it.s, sz = it.sz;
     const int cut = sz / 2;
     const double rho = fabs(2.0 * ew[s + cut - 1]);
     const double tol = 8.0 * kEps * fmax(fabs(md[s]), fabs(md[s + sz - 1]));
@@ -800,13 +874,13 @@ __device__ __forceinline__ void secular_side(const double2* __restrict__ dz, int
 
 // One group of 64/L roots of one merge.  Returns the group's secular
 // iteration count (summed over the wave).
-__device__ __forceinline__ int secular_group(int n, int s, int sz, int K, int L, int grp,
-                                             double rho, const double* __restrict__ ad,
+__device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, int K, int L,
+                                             int grp, double rho, const double* __restrict__ ad,
                                              const double* __restrict__ az,
                                              const double2* __restrict__ adz,
                                              double* __restrict__ rlam, double2* __restrict__ rs,
-                                             double* __restrict__ rn1, double* __restrict__ rn2,
-                                             int* info) {
+                                             double* __restrict__ rn1,
+                                             double* __restrict__ rn2) {
     const int lane = threadIdx.x & (kWave - 1);
     const int i = grp * (kWave / L) + lane / L;  // this lane's root
     const int c = lane & (L - 1);
@@ -862,7 +936,7 @@ __device__ __forceinline__ int secular_group(int n, int s, int sz, int K, int L,
         rn1[s + i] = r.near1;
         rn2[s + i] = r.near2;
         iters = r.iters;
-        if (r.info) report_failure(info, n, s, sz);
+        if (r.info) report_item_failure(fs, sz);
     }
     return iters;
 }
@@ -873,11 +947,11 @@ __device__ __forceinline__ int secular_group(int n, int s, int sz, int K, int L,
 // capped, each wave walking tasks with a grid stride.
 template <int WL>
 __global__ __launch_bounds__(kWave * WL) void k_secular(
-    int n, int depth, int slots, int kstream, const double* __restrict__ ew, const int* counts,
+    Geo g, int merges, int slots, int kstream, const double* __restrict__ ew, const int* counts,
     const double* __restrict__ ad, const double* __restrict__ az,
     const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
-    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
-    const int merges = 1 << depth;
+    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw,
+    const int* __restrict__ moff, int* info) {
     const int ntask = merges * slots;
     const int nwave = gridDim.x * WL;
     const int lane = threadIdx.x & (kWave - 1);
@@ -888,13 +962,13 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
         const int L = secular_lanes(K, merges);
         const int rpw = kWave / L;  // roots per wave
         if (slot * rpw >= K || K >= kstream) continue;  // idle slot / streamed merge
-        int s, sz;
-        node_geom(n, depth, merge, &s, &sz);
+        const Item it = g.item(merge);
+        const int s = it.s, sz = it.sz;
+        const FailSite fs = fail_site(info, moff, g.n, it);
         const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
         int iters = 0;
         for (int grp = slot; grp * rpw < K; grp += slots)
-            iters += secular_group(n, s, sz, K, L, grp, rho, ad, az, adz, rlam, rs, rn1, rn2,
-                                   info);
+            iters += secular_group(fs, s, sz, K, L, grp, rho, ad, az, adz, rlam, rs, rn1, rn2);
         for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
         if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
     }
@@ -1042,23 +1116,24 @@ __device__ __forceinline__ int stream_lanes(int K, int merges, int groups, int W
 template <int W>
 __global__ __launch_bounds__(kWave * W) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_secular_stream(
-    int n, int depth, int kstream, const double* __restrict__ ew, const int* counts,
+    Geo g, int merges, int kstream, const double* __restrict__ ew, const int* counts,
     const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
-    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw, int* info) {
+    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw,
+    const int* __restrict__ moff, int* info) {
     __shared__ double part[W][6][kWave];
     __shared__ double2 stage[W][kStageChunk];
     __shared__ double pass_org[kWave], pass_tau[kWave];
     __shared__ int pass_exlo[kWave], pass_nex[kWave];
     __shared__ int pass_live;
-    const int merges = 1 << depth;
     const int wave = uniform(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     double2* slot = stage[wave];
     for (int mi = 0; mi < merges; mi++) {
         const int merge = (int)((blockIdx.x + (unsigned)mi) % (unsigned)merges);
         const int K = uniform(counts[kCountStride * merge]);
         if (K < kstream) continue;
-        int s, sz;
-        node_geom(n, depth, merge, &s, &sz);
+        const Item it = g.item(merge);
+        const int s = it.s, sz = it.sz;
+        const FailSite fs = fail_site(info, moff, g.n, it);
         const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
         const double2* __restrict__ dz = adz + s;
         const PoleView E{dz};
@@ -1133,7 +1208,7 @@ void k_secular_stream(
                         rs[s + i] = make_double2(dz[r.org].x, r.tau);
                         rn1[s + i] = r.near1;
                         rn2[s + i] = r.near2;
-                        if (r.info) report_failure(info, n, s, sz);
+                        if (r.info) report_item_failure(fs, sz);
                         iters += r.iters;
                     }
                     i = -1;
@@ -1154,7 +1229,7 @@ void k_secular_stream(
 // the two pole-adjacent entries are root j's near1/near2 (dlaed9_br.f:272-331).
 template <int WL>
 __global__ __launch_bounds__(kGroup * WL) void k_wprod(
-    int n, int depth, int bpg, const int* __restrict__ counts, const double* __restrict__ ad,
+    Geo geo, int bpg, const int* __restrict__ counts, const double* __restrict__ ad,
     const double* __restrict__ az, const double2* __restrict__ rs,
     const double* __restrict__ rn1, const double* __restrict__ rn2, double2* __restrict__ pw) {
     __shared__ double red[WL > 1 ? WL : 1][kGroup];
@@ -1162,8 +1237,9 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     const int K = uniform(counts[kCountStride * merge]);
     const int g0 = g * kGroup;
     if (K <= 2 || g0 >= K) return;
-    int s, sz;
-    node_geom(n, depth, merge, &s, &sz);
+    const Item it = geo.item(merge);
+    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
+    const int s = it.s;
     const int lane = threadIdx.x & (kGroup - 1);
     const int c = uniform(threadIdx.x / kGroup);
     const int W = waves_for(K, WL);
@@ -1233,7 +1309,7 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
 // x_i = w_i / delta_j(i), row_r(j) = sum_i q_r(i) x_i / ||x||.
 template <int WL>
 __global__ __launch_bounds__(kGroup * WL) void k_rows(
-    int n, int depth, int bpg, const int* __restrict__ counts, const double2* __restrict__ aq,
+    Geo geo, int bpg, const int* __restrict__ counts, const double2* __restrict__ aq,
     const double2* __restrict__ pw, const double2* __restrict__ rs,
     const double* __restrict__ rn1, const double* __restrict__ rn2, double* __restrict__ qo1,
     double* __restrict__ qo2) {
@@ -1242,8 +1318,9 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
     const int K = uniform(counts[kCountStride * merge]);
     const int g0 = g * kGroup;
     if (g0 >= K) return;
-    int s, sz;
-    node_geom(n, depth, merge, &s, &sz);
+    const Item it = geo.item(merge);
+    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
+    const int s = it.s;
     const int lane = threadIdx.x & (kGroup - 1);
     const int c = uniform(threadIdx.x / kGroup);
     if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172)
@@ -1332,15 +1409,17 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
 // values, dlaed7_br.f:146, and the boundary-row persist, 155-160) as a 3-way
 // merge of three sorted lists: roots, Givens-deflated, tiny-deflated.
 __global__ __launch_bounds__(kThreads) void k_finalize(
-    int n, int depth, int bpm, const int* __restrict__ counts, const double* __restrict__ rlam,
+    Geo g, int bpm, const int* __restrict__ counts, const double* __restrict__ rlam,
     const double* __restrict__ qo1, const double* __restrict__ qo2, const double* __restrict__ gd,
     const double* __restrict__ gq1, const double* __restrict__ gq2, const double* __restrict__ td,
-    const double* __restrict__ tq1, const double* __restrict__ tq2, double* __restrict__ od,
-    double* __restrict__ olo, double* __restrict__ ohi, int root, const double* __restrict__ scal,
-    const int* __restrict__ info, int* info_user) {
+    const double* __restrict__ tq1, const double* __restrict__ tq2, double* __restrict__ out_final,
+    double* __restrict__ od, double* __restrict__ olo, double* __restrict__ ohi,
+    const double* __restrict__ scal, const int* __restrict__ info, int* info_user) {
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    int s, sz;
-    node_geom(n, depth, merge, &s, &sz);
+    const Item it = g.item(merge);
+    const int s = it.s, sz = it.sz;
+    const bool root = (it.flags & kItemRoot) != 0;
+    // one matrix: the root merge's last kernel hands INFO to the caller
     if (root && blockIdx.x == 0 && threadIdx.x == 0 && info_user) *info_user = *info;
     const int idx = part * kThreads + threadIdx.x;
     if (idx >= sz) return;
@@ -1367,7 +1446,7 @@ __global__ __launch_bounds__(kThreads) void k_finalize(
         q2 = tq2[s + t];
     }
     if (root) {
-        od[s + rank] = x * scal[1];
+        out_final[s + rank] = x * scal[2 * it.mat + 1];
     } else {
         od[s + rank] = x;
         olo[s + rank] = q1;

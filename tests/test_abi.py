@@ -82,3 +82,68 @@ def test_device_entry_rejects_bad_arguments_without_launching():
     assert L.brdc_dstebr_device_ex(100, 0x1000, 0x2000, 0x100000, 1 << 30, None, None,
                                    ctypes.byref(o)) == 3
     assert L.brdc_dstebr_device(0, None, None, None, 0, None, None) == 0
+
+
+def _plan(L, sizes, leaf=0):
+    import numpy as np
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    p = ctypes.c_void_p()
+    o = _lib.options(leaf)
+    st = L.brdc_batch_plan_create(len(sizes), sizes.ctypes.data_as(_lib._ip), ctypes.byref(o),
+                                  ctypes.byref(p))
+    return st, p
+
+
+def test_batch_plan_is_every_matrix_tree():
+    """The flattened forest holds each matrix's own tree (the tree of the
+    single-matrix entry): leaves = sum 2^levels_b, merges = sum (2^levels_b - 1),
+    heights = max levels_b, over the same floor/ceil halving."""
+    import numpy as np
+    from brdc.generators import batched_sizes
+    L = _lib.load()
+    for leaf in (8, 16, 32, 64):
+        sizes = np.concatenate([[0, 1, 2, 3, leaf, leaf + 1, 2 * leaf + 1, 1000],
+                                batched_sizes(300, 32, 4096, 99)]).astype(np.int32)
+        st, p = _plan(L, sizes, leaf)
+        assert st == 0
+        out = np.zeros(7, dtype=np.int32)
+        assert L.brdc_batch_plan_stats(p, out.ctypes.data_as(_lib._ip), 7) == 7
+        lev = []
+        for n in sizes:
+            if n == 0:
+                continue
+            k = 0
+            while -(-n // (1 << k)) > leaf:
+                k += 1
+            lev.append(k)
+        lev = np.array(lev)
+        assert out[0] == sizes.sum() and out[1] == len(sizes)
+        assert out[2] == np.sum(2 ** lev) and out[4] == np.sum(2 ** lev - 1)
+        assert out[3] == lev.max() and out[5] <= leaf and out[6] == leaf
+        nb = ctypes.c_size_t(0)
+        assert L.brdc_batch_workspace(p, ctypes.byref(nb)) == 0
+        assert 0 < nb.value < 400 * int(sizes.sum()) + 64 * out[2] + 4096 * len(sizes)
+        L.brdc_batch_plan_destroy(p)
+
+
+def test_batch_entry_rejects_bad_arguments_without_launching():
+    import numpy as np
+    L = _lib.load()
+    st, p = _plan(L, [10, -1, 5])
+    assert st == 3 and not p.value
+    st, p = _plan(L, [10, 20], leaf=65)
+    assert st == 3 and not p.value
+    st, p = _plan(L, [10, 20])
+    assert st == 0 and p.value
+    nb = ctypes.c_size_t(0)
+    L.brdc_batch_workspace(p, ctypes.byref(nb))
+    assert L.brdc_batch_upload(p, None, nb.value, None) == 2
+    assert L.brdc_batch_upload(p, 0x100000, 16, None) == 4
+    assert L.brdc_batch_solve(p, 0x1000, 0x2000, 0x100000, nb.value, None, None) == 2
+    assert L.brdc_batch_solve(p, 0x1000, 0x2000, 0x100010, nb.value, None, 0x3000) == 4
+    assert L.brdc_batch_solve(None, 0x1000, 0x2000, 0x100000, nb.value, None, 0x3000) == 2
+    L.brdc_batch_plan_destroy(p)
+    st, p = _plan(L, np.zeros(0, dtype=np.int32))
+    assert st == 0
+    assert L.brdc_batch_solve(p, None, None, None, 0, None, None) == 0  # empty batch
+    L.brdc_batch_plan_destroy(p)
