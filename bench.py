@@ -113,6 +113,93 @@ def copy_probe(mib):
                       "launches": 3}), flush=True)
 
 
+def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True):
+    """The batched workload (2048 tridiagonals, sizes log-uniform in [32, 4096],
+    D, E ~ N(0,1)) as one flattened merge forest, leaf bound autotuned;
+    reported beside the plain per-matrix GPU path and the CPU oracle."""
+    import torch
+    import brdc
+    from brdc import autotune
+    from brdc.generators import batched_workload, pack_batch
+    dev = torch.device("cuda:0")
+    sizes, mats = batched_workload()
+    d_cat, e_cat, off = pack_batch(mats)
+    d_dev = torch.from_numpy(d_cat).to(dev)
+    e_dev = torch.from_numpy(e_cat).to(dev)
+    t0 = time.perf_counter()
+    tune = autotune.tune_batched(d_dev, e_dev, sizes, reps=3)
+    tune_s = time.perf_counter() - t0
+    bs = brdc.BatchSolver(sizes, device=dev, leaf_max=tune["leaf_max"])
+    gs = brdc.GraphSolve(bs, d_dev, e_dev)
+    for _ in range(warmup):
+        gs.run()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        gs.run()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    w = gs.w.cpu().numpy()
+    infos = bs.info.cpu().numpy()[:bs.nmat]
+    ws = bs.split(w)
+    inv = max(invariants_err(ws[i], *mats[i]) for i in range(len(mats)))
+    # the plain per-matrix GPU path on the same matrices (one solve per matrix)
+    solvers = {}
+    dd = [torch.from_numpy(d).to(dev) for d, _ in mats]
+    ee = [torch.from_numpy(e).to(dev) for _, e in mats]
+    outs = [torch.empty_like(x) for x in dd]
+    for n in set(int(x) for x in sizes):
+        solvers[n] = brdc.Solver(n, device=dev, leaf_max=tune["leaf_max"])
+    for i in range(min(32, len(mats))):
+        outs[i].copy_(dd[i])
+        solvers[int(sizes[i])].solve(outs[i], ee[i])
+    torch.cuda.synchronize()
+    a.record()
+    for i in range(len(mats)):
+        outs[i].copy_(dd[i])
+        solvers[int(sizes[i])].solve(outs[i], ee[i])
+    b.record()
+    torch.cuda.synchronize()
+    per_matrix_ms = a.elapsed_time(b)
+    per_matrix_diff = max(rel_err(ws[i], outs[i].cpu().numpy()) for i in range(0, len(mats), 64))
+    rec = {"workload": "2048 tridiagonals, n = round(exp(U(ln 32, ln 4096))), D,E ~ N(0,1), "
+                       "one flattened merge forest per launch sequence (hipGraph replay)",
+           "matrices": len(mats), "rows": int(off[-1]), "leaf_max": tune["leaf_max"],
+           "autotune_ms_per_batch": {str(k): round(v, 4) for k, v in tune["ms"].items()},
+           "autotune_s": round(tune_s, 2),
+           "ms_per_batch": round(ms, 4), "solves_per_s": round(len(mats) / (ms * 1e-3), 1),
+           "per_matrix_gpu_ms_per_batch": round(per_matrix_ms, 3),
+           "per_matrix_gpu_solves_per_s": round(len(mats) / (per_matrix_ms * 1e-3), 1),
+           "speedup_vs_per_matrix_gpu": round(per_matrix_ms / ms, 2),
+           "info_nonzero": int(np.count_nonzero(infos)), "worst_invariants": inv,
+           "max_rel_diff_vs_per_matrix_gpu": per_matrix_diff}
+    if want_cpu:
+        import oracle
+        if not os.path.exists(oracle.LIB_PATH):
+            oracle.build()
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        # a sample of whole matrices in batch order, within the time budget
+        done, t_cpu, worst = 0, 0.0, 0.0
+        while done < len(mats) and t_cpu < cpu_budget_s:
+            d, e = mats[done]
+            t1 = time.perf_counter()
+            ref, _ = oracle.dstebr(d, e, nthreads=cores)
+            t_cpu += time.perf_counter() - t1
+            worst = max(worst, rel_err(ws[done], ref))
+            done += 1
+        rec["cpu_baseline"] = {"value": round(done / t_cpu, 1), "unit": "solves/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"oracle BR restatement on the first {done} matrices "
+                                         f"of the batch ({int(sum(sizes[:done]))} rows)"}
+        rec["max_rel_err_vs_cpu_oracle"] = worst
+    rec["pass"] = bool(rec["info_nonzero"] == 0 and inv < 1e-10 and per_matrix_diff < 1e-10
+                       and rec.get("max_rel_err_vs_cpu_oracle", 0.0) <= 1e-10)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying a captured graph")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-workload section")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, three "
                          "times (calibrates the HBM byte counters of a profiler run)")
@@ -304,6 +393,9 @@ def main():
     }
     if cpu_rec:
         line["speedup_vs_cpu_baseline"] = round(cpu_rec["value"] / value, 2) if value > 0 else None
+    if not args.no_batched:
+        line["batched"] = bench_batched(max(3, args.steps // 2), max(1, args.warmup),
+                                        want_cpu=not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
 
 
