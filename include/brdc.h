@@ -162,6 +162,32 @@ int brdc_batch_solve(const brdc_batch_plan *plan, double *d_dev, const double *e
  * largest leaf, leaf bound).  Returns the number written or a negative status. */
 int brdc_batch_plan_stats(const brdc_batch_plan *plan, int *out, int nout);
 
+/*
+ * Backward of the eigenvalues (no counterpart in the reference, whose solver
+ * is values-only).  Given D, E, the ascending eigenvalues W from the solve and
+ * an upstream gradient G (all device, FP64), writes
+ *   DD[j] = sum_i G[i] v_ij^2             (dL/dD_j)
+ *   DE[j] = 2 sum_i G[i] v_ij v_i,j+1     (dL/dE_j; DE[n-1] = 0)
+ * with each unit eigenvector v_i recomputed at W[i] by twisted factorization
+ * (nothing is stored by the forward pass).  The scratch buffer (caller-owned,
+ * 256-byte aligned) holds per-wave pivot arrays and per-task partial sums;
+ * the sums over tasks are added in a fixed order, so results are
+ * reproducible.  Asynchronous on stream; returns a status for the launch.
+ */
+int brdc_dstebr_grad_workspace(int n, size_t *bytes);
+int brdc_dstebr_grad_device(int n, const double *d_dev, const double *e_dev,
+                            const double *w_dev, const double *g_dev, double *dd_dev,
+                            double *de_dev, void *scratch, size_t scratch_bytes, void *stream);
+
+/* Batched backward over a plan's concatenated rows (same layout as
+ * brdc_batch_solve; DE is 0 at each matrix's last row).  workspace: a
+ * workspace the plan was uploaded to (its tables are read). */
+int brdc_batch_grad_workspace(const brdc_batch_plan *plan, size_t *bytes);
+int brdc_batch_grad(const brdc_batch_plan *plan, const double *d_dev, const double *e_dev,
+                    const double *w_dev, const double *g_dev, double *dd_dev, double *de_dev,
+                    const void *workspace, size_t workspace_bytes, void *scratch,
+                    size_t scratch_bytes, void *stream);
+
 /* Human-readable name of a brdc_status. */
 const char *brdc_status_string(int status);
 

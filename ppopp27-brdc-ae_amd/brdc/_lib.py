@@ -21,7 +21,9 @@ EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
            "brdc_dstebr_device_workspace_ex", "brdc_dstebr_device", "brdc_dstebr_device_ex",
            "brdc_merge_stats", "brdc_status_string", "brdc_profile_begin", "brdc_profile_end",
            "brdc_batch_plan_create", "brdc_batch_plan_destroy", "brdc_batch_workspace",
-           "brdc_batch_upload", "brdc_batch_solve", "brdc_batch_plan_stats")
+           "brdc_batch_upload", "brdc_batch_solve", "brdc_batch_plan_stats",
+           "brdc_dstebr_grad_workspace", "brdc_dstebr_grad_device", "brdc_batch_grad_workspace",
+           "brdc_batch_grad")
 
 KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
                   "finalize")
@@ -86,6 +88,16 @@ def load() -> ctypes.CDLL:
     L.brdc_batch_solve.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp]
     L.brdc_batch_plan_stats.restype = ctypes.c_int
     L.brdc_batch_plan_stats.argtypes = [_vp, _ip, ctypes.c_int]
+    L.brdc_dstebr_grad_workspace.restype = ctypes.c_int
+    L.brdc_dstebr_grad_workspace.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+    L.brdc_dstebr_grad_device.restype = ctypes.c_int
+    L.brdc_dstebr_grad_device.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          ctypes.c_size_t, _vp]
+    L.brdc_batch_grad_workspace.restype = ctypes.c_int
+    L.brdc_batch_grad_workspace.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.brdc_batch_grad.restype = ctypes.c_int
+    L.brdc_batch_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp,
+                                  ctypes.c_size_t, _vp]
     _lib = L
     return L
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/brdc.h"
+#include "brdc_grad.h"
 #include "brdc_kernels.h"
 
 namespace brdc {
@@ -65,7 +66,7 @@ struct Layout {
     size_t pw, qo1, qo2;
     size_t P, rend, fl;
     size_t counts, info;
-    size_t plan, moff;  // batch only
+    size_t plan, moff, tpre, poff;  // batch only
     size_t total;
 };
 
@@ -114,6 +115,8 @@ Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items) 
     L.info = take(sizeof(int));
     L.plan = plan_items ? take(plan_items * sizeof(int4)) : 0;
     L.moff = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
+    L.tpre = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
+    L.poff = plan_items ? take(((size_t)nmat + 1) * sizeof(long long)) : 0;
     L.total = off;
     return L;
 }
@@ -400,6 +403,9 @@ struct HostLevel {
 struct brdc_batch_plan {
     int nmat = 0, rows = 0, leaf = 0;
     std::vector<int> moff;           // nmat + 1
+    std::vector<int> tprefix;        // gradient tasks (64 eigenvalues) before each matrix
+    std::vector<long long> poff;     // gradient partial-sum offset of each matrix
+    int rows_max = 0;
     std::vector<int4> items;         // leaves, then every level's merges
     int nleaves = 0, leaf_rows = 0;
     std::vector<brdc::HostLevel> levels;  // height 1 upward
@@ -409,6 +415,58 @@ struct brdc_batch_plan {
 
 namespace brdc {
 namespace {
+
+// Gradient scratch: two per-wave pivot arrays of rows_max x 64 doubles for
+// each of `waves` waves, then the per-task partial sums.
+constexpr size_t kGradScratchCap = (size_t)4 << 30;  // bound on the pivot arrays
+
+struct GradShape {
+    int waves;
+    long long slot;  // doubles per pivot array per wave
+    size_t a, b, partial, total;
+};
+
+GradShape grad_shape(int ntask, int rows_max, long long partial_doubles) {
+    GradShape G{};
+    G.slot = (long long)std::max(rows_max, 1) * kWave;
+    const size_t per_wave = 2 * (size_t)G.slot * sizeof(double);
+    int cap = (int)std::max<size_t>(kGradWaves, kGradScratchCap / per_wave);
+    cap = std::min(cap, 4096) / kGradWaves * kGradWaves;
+    G.waves = std::max(kGradWaves, std::min((ntask + kGradWaves - 1) / kGradWaves * kGradWaves,
+                                            cap));
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = align_up(off + bytes);
+        return at;
+    };
+    G.a = take((size_t)G.waves * G.slot * sizeof(double));
+    G.b = take((size_t)G.waves * G.slot * sizeof(double));
+    G.partial = take((size_t)std::max<long long>(partial_doubles, 1) * sizeof(double));
+    G.total = off;
+    return G;
+}
+
+int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const long long* poff,
+                 int ntask, int rows_max, long long partial_doubles, const double* d,
+                 const double* e, const double* w, const double* g, double* dd, double* de,
+                 void* scratch, hipStream_t st) {
+    const GradShape G = grad_shape(ntask, rows_max, partial_doubles);
+    double* A = at<double>(scratch, G.a);
+    double* B = at<double>(scratch, G.b);
+    double* part = at<double>(scratch, G.partial);
+    const int n_one = moff ? 0 : rows;
+    if (ntask > 0) {
+        hipLaunchKernelGGL(k_grad_vectors, dim3(G.waves / kGradWaves), dim3(kWave * kGradWaves),
+                           0, st, nmat, moff, tprefix, poff, ntask, n_one, d, e, w, g, A, B,
+                           G.slot, part);
+        BRDC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
+                           moff, tprefix, poff, n_one, (const double*)part, dd, de);
+        BRDC_LAUNCH_CHECK();
+    }
+    return BRDC_STATUS_SUCCESS;
+}
 
 Schedule batch_schedule(const brdc_batch_plan& p, void* ws) {
     Schedule S;
@@ -628,10 +686,16 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
     std::vector<int> lev((size_t)nmat, 0);
     int H = 0;
     p->moff[0] = 0;
+    p->tprefix.assign((size_t)nmat + 1, 0);
+    p->poff.assign((size_t)nmat + 1, 0);
     for (int b = 0; b < nmat; b++) {
         p->moff[b + 1] = p->moff[b] + sizes[b];
         lev[b] = (sizes[b] > 0) ? tree_levels(sizes[b], p->leaf) : -1;
         H = std::max(H, lev[b]);
+        const int tasks = (sizes[b] + kWave - 1) / kWave;
+        p->tprefix[b + 1] = p->tprefix[b] + tasks;
+        p->poff[b + 1] = p->poff[b] + (long long)tasks * 2 * sizes[b];
+        p->rows_max = std::max(p->rows_max, sizes[b]);
     }
     // leaves of every matrix (a matrix that fits one leaf is solved there)
     for (int b = 0; b < nmat; b++) {
@@ -699,6 +763,11 @@ int brdc_batch_upload(const brdc_batch_plan* plan, void* workspace, size_t works
         return BRDC_STATUS_HIP_ERROR;
     if (hipMemcpyAsync(at<int>(workspace, L.moff), plan->moff.data(),
                        plan->moff.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(at<int>(workspace, L.tpre), plan->tprefix.data(),
+                       plan->tprefix.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(at<long long>(workspace, L.poff), plan->poff.data(),
+                       plan->poff.size() * sizeof(long long), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return BRDC_STATUS_HIP_ERROR;
     return BRDC_STATUS_SUCCESS;
@@ -714,6 +783,57 @@ int brdc_batch_solve(const brdc_batch_plan* plan, double* d_dev, const double* e
         return BRDC_STATUS_INVALID_WORKSPACE;
     return enqueue(batch_schedule(*plan, workspace), d_dev, e_dev, workspace, L,
                    static_cast<hipStream_t>(stream), info_dev, nullptr);
+}
+
+int brdc_dstebr_grad_workspace(int n, size_t* bytes) {
+    if (!bytes) return BRDC_STATUS_INVALID_POINTER;
+    if (n < 0) return BRDC_STATUS_INVALID_VALUE;
+    const int tasks = (n + kWave - 1) / kWave;
+    *bytes = grad_shape(tasks, n, (long long)tasks * 2 * n).total;
+    return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
+                            const double* w_dev, const double* g_dev, double* dd_dev,
+                            double* de_dev, void* scratch, size_t scratch_bytes, void* stream) {
+    if (n < 0) return BRDC_STATUS_INVALID_VALUE;
+    if (n == 0) return BRDC_STATUS_SUCCESS;
+    if (!d_dev || !e_dev || !w_dev || !g_dev || !dd_dev || !de_dev || !scratch)
+        return BRDC_STATUS_INVALID_POINTER;
+    const int tasks = (n + kWave - 1) / kWave;
+    const long long partial = (long long)tasks * 2 * n;
+    if (scratch_bytes < grad_shape(tasks, n, partial).total ||
+        (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
+        return BRDC_STATUS_INVALID_WORKSPACE;
+    return enqueue_grad(n, 1, nullptr, nullptr, nullptr, tasks, n, partial, d_dev, e_dev, w_dev,
+                        g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
+}
+
+int brdc_batch_grad_workspace(const brdc_batch_plan* plan, size_t* bytes) {
+    if (!plan || !bytes) return BRDC_STATUS_INVALID_POINTER;
+    *bytes = grad_shape(plan->tprefix.back(), plan->rows_max, plan->poff.back()).total;
+    return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const double* e_dev,
+                    const double* w_dev, const double* g_dev, double* dd_dev, double* de_dev,
+                    const void* workspace, size_t workspace_bytes, void* scratch,
+                    size_t scratch_bytes, void* stream) {
+    if (!plan) return BRDC_STATUS_INVALID_POINTER;
+    if (plan->rows == 0) return BRDC_STATUS_SUCCESS;
+    if (!d_dev || !e_dev || !w_dev || !g_dev || !dd_dev || !de_dev || !workspace || !scratch)
+        return BRDC_STATUS_INVALID_POINTER;
+    const Layout& L = plan->layout;
+    const int ntask = plan->tprefix.back();
+    const long long partial = plan->poff.back();
+    if (workspace_bytes < L.total || scratch_bytes < grad_shape(ntask, plan->rows_max,
+                                                                  partial).total ||
+        (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
+        return BRDC_STATUS_INVALID_WORKSPACE;
+    void* ws = const_cast<void*>(workspace);
+    return enqueue_grad(plan->rows, plan->nmat, at<int>(ws, L.moff), at<int>(ws, L.tpre),
+                        at<long long>(ws, L.poff), ntask, plan->rows_max, partial, d_dev, e_dev,
+                        w_dev, g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

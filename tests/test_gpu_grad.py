@@ -1,0 +1,166 @@
+"""GPU checks of the hand-written backward (dL/dD_j = sum_i g_i v_ij^2,
+dL/dE_j = 2 sum_i g_i v_ij v_i,j+1, eigenvectors recomputed by twisted
+factorization) through the C ABI and torch.autograd: against a dense FP64
+eigh gradient, central finite differences (h = 1e-6), the Euler identity of
+the degree-1 homogeneous eigenvalues, and between the batched and
+single-matrix entries.  The reference is values-only; this path is new."""
+import numpy as np
+import pytest
+
+from brdc.generators import generate, pack_batch, seed_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import brdc
+    brdc._lib.load()
+    return torch
+
+
+def dense_grad(d, e, g):
+    """dL/dD, dL/dE from a dense FP64 eigendecomposition (numpy LAPACK)."""
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    lam, V = np.linalg.eigh(T)
+    return (V * V) @ g, 2.0 * (V[:-1] * V[1:]) @ g, lam
+
+
+def gpu_grad(torch, d, e, g):
+    from brdc.autograd import eigvalsh_tridiagonal
+    dev = torch.device("cuda:0")
+    dt = torch.tensor(d, device=dev, requires_grad=True)
+    et = torch.tensor(e, device=dev, requires_grad=True)
+    w = eigvalsh_tridiagonal(dt, et)
+    (w * torch.tensor(g, device=dev)).sum().backward()
+    return dt.grad.cpu().numpy(), et.grad.cpu().numpy(), w.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("kind,n", [("normal", 2), ("normal", 3), ("normal", 64), ("normal", 65),
+                                    ("normal", 500), ("uniform", 1024), ("toeplitz", 300),
+                                    ("normal", 2000)])
+def test_grad_matches_dense_eigh(gpu, kind, n):
+    d, e = generate(kind, n, seed_for(5, n))
+    g = np.random.default_rng(n).standard_normal(n)
+    gd, ge, w = gpu_grad(gpu, d, e, g)
+    rd, re, lam = dense_grad(d, e, g)
+    scale = np.max(np.abs(g))
+    assert np.max(np.abs(w - lam)) <= 1e-12 * np.max(np.abs(lam))
+    assert np.max(np.abs(gd - rd)) <= 1e-10 * scale, np.max(np.abs(gd - rd))
+    assert np.max(np.abs(ge - re)) <= 1e-10 * scale, np.max(np.abs(ge - re))
+
+
+def test_grad_matches_central_differences(gpu):
+    """L = g . lambda(D, E); central differences with h = 1e-6 on every entry
+    of a 40 x 40 matrix, eigenvalues from the GPU solve itself."""
+    import brdc
+    torch = gpu
+    n = 40
+    d, e = generate("normal", n, seed_for(5, n, 1))
+    g = np.random.default_rng(1).standard_normal(n)
+    gd, ge, _ = gpu_grad(torch, d, e, g)
+    h = 1e-6
+
+    def loss(dd, ee):
+        w, info = brdc.dstebr(dd, ee)
+        assert info == 0
+        return float(g @ w)
+    fd_d = np.array([(loss(d + h * (np.arange(n) == j), e) - loss(d - h * (np.arange(n) == j), e))
+                     / (2 * h) for j in range(n)])
+    fd_e = np.array([(loss(d, e + h * (np.arange(n - 1) == j)) -
+                      loss(d, e - h * (np.arange(n - 1) == j))) / (2 * h) for j in range(n - 1)])
+    assert np.max(np.abs(fd_d - gd)) <= 1e-7 * np.max(np.abs(g))
+    assert np.max(np.abs(fd_e - ge)) <= 1e-7 * np.max(np.abs(g))
+
+
+def test_grad_identities_at_n16384(gpu):
+    """n = 16384 (the differentiated single matrix): sum_j dD_j = sum_i g_i
+    (unit eigenvectors) and sum_j D_j dD_j + E_j dE_j = sum_i g_i lambda_i
+    (eigenvalues are homogeneous of degree 1 in (D, E))."""
+    n = 16384
+    d, e = generate("normal", n, seed_for(5, n))
+    g = np.random.default_rng(2).standard_normal(n)
+    gd, ge, w = gpu_grad(gpu, d, e, g)
+    s = np.sum(np.abs(g))
+    assert abs(gd.sum() - g.sum()) <= 1e-11 * s
+    assert abs(d @ gd + e @ ge - g @ w) <= 1e-11 * s * np.max(np.abs(w))
+    # a spread of single eigenvalues: g = unit vector -> dD = v_i^2 sums to 1,
+    # is nonnegative, and dE = 2 v_i v_i+1
+    from brdc.autograd import grad_single
+    torch = gpu
+    dev = torch.device("cuda:0")
+    dt, et = torch.tensor(d, device=dev), torch.tensor(e, device=dev)
+    wt = torch.tensor(w, device=dev)
+    for i in (0, 4097, n // 2, n - 1):
+        gi = torch.zeros(n, dtype=torch.float64, device=dev)
+        gi[i] = 1.0
+        dd, de = grad_single(dt, et, wt, gi)
+        dd = dd.cpu().numpy()
+        de = de.cpu().numpy()
+        assert abs(dd.sum() - 1.0) <= 1e-12 and dd.min() >= 0.0
+        v = np.sqrt(dd)
+        assert np.max(np.abs(np.abs(de) - 2 * v[:-1] * v[1:])) <= 1e-12
+
+
+def test_batched_grad_equals_single_grads(gpu):
+    import brdc
+    from brdc.autograd import eigvalsh_tridiagonal_batched, grad_single
+    torch = gpu
+    dev = torch.device("cuda:0")
+    sizes = [1, 2, 63, 64, 65, 1024, 1024, 3000]
+    mats = [generate("normal", n, seed_for(10, n, b)) for b, n in enumerate(sizes)]
+    d_cat, e_cat, off = pack_batch(mats)
+    bs = brdc.BatchSolver(sizes, device=dev)
+    dt = torch.tensor(d_cat, device=dev, requires_grad=True)
+    et = torch.tensor(e_cat, device=dev, requires_grad=True)
+    g = torch.tensor(np.random.default_rng(3).standard_normal(d_cat.size), device=dev)
+    w = eigvalsh_tridiagonal_batched(dt, et, bs)
+    (w * g).sum().backward()
+    gd, ge = dt.grad.cpu().numpy(), et.grad.cpu().numpy()
+    wn = w.detach()
+    for b, (d, e) in enumerate(mats):
+        o, n = int(off[b]), sizes[b]
+        sd, se = grad_single(torch.tensor(d, device=dev), torch.tensor(e, device=dev),
+                             wn[o:o + n].contiguous(), g[o:o + n].contiguous())
+        # same tasks and the same summation order per matrix: identical bits
+        np.testing.assert_array_equal(gd[o:o + n], sd.cpu().numpy())
+        np.testing.assert_array_equal(ge[o:o + n - 1], se.cpu().numpy())
+        assert ge[o + n - 1] == 0.0
+        rd, re, _ = dense_grad(d, e, g[o:o + n].cpu().numpy())
+        assert np.max(np.abs(gd[o:o + n] - rd)) <= 1e-10 * np.max(np.abs(g.cpu().numpy()))
+
+
+def test_differentiated_workload_shapes(gpu):
+    """The differentiated configuration: 256 matrices at n = 1024 plus one at
+    n = 16384 in one batch, upstream g ~ N(0,1): every matrix obeys the two
+    identities; two of the small ones against dense eigh."""
+    import brdc
+    from brdc.autograd import eigvalsh_tridiagonal_batched
+    torch = gpu
+    dev = torch.device("cuda:0")
+    sizes = [1024] * 256 + [16384]
+    mats = [generate("normal", n, seed_for(5, n, b)) for b, n in enumerate(sizes)]
+    d_cat, e_cat, off = pack_batch(mats)
+    bs = brdc.BatchSolver(sizes, device=dev)
+    dt = torch.tensor(d_cat, device=dev, requires_grad=True)
+    et = torch.tensor(e_cat, device=dev, requires_grad=True)
+    g = np.random.default_rng(4).standard_normal(d_cat.size)
+    w = eigvalsh_tridiagonal_batched(dt, et, bs)
+    (w * torch.tensor(g, device=dev)).sum().backward()
+    gd, ge, wv = dt.grad.cpu().numpy(), et.grad.cpu().numpy(), w.detach().cpu().numpy()
+    for b, (d, e) in enumerate(mats):
+        o, n = int(off[b]), sizes[b]
+        gb = g[o:o + n]
+        s = np.sum(np.abs(gb))
+        assert abs(gd[o:o + n].sum() - gb.sum()) <= 1e-11 * s
+        assert abs(d @ gd[o:o + n] + e @ ge[o:o + n - 1] - gb @ wv[o:o + n]) <= \
+            1e-11 * s * np.max(np.abs(wv[o:o + n]))
+    for b in (0, 131):
+        d, e = mats[b]
+        o, n = int(off[b]), sizes[b]
+        rd, re, _ = dense_grad(d, e, g[o:o + n])
+        assert np.max(np.abs(gd[o:o + n] - rd)) <= 1e-10 * np.max(np.abs(g))
+        assert np.max(np.abs(ge[o:o + n - 1] - re)) <= 1e-10 * np.max(np.abs(g))
