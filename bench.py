@@ -200,6 +200,99 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True):
     return rec
 
 
+def bench_differentiated(steps, warmup):
+    """The differentiated workload: 256 tridiagonals at n = 1024 plus one at
+    n = 16384, D, E ~ N(0,1), upstream g ~ N(0,1); forward (one batched
+    solve) + hand-written backward through torch.autograd."""
+    import torch
+    import brdc
+    from brdc.autograd import eigvalsh_tridiagonal_batched, grad_batched
+    from brdc.generators import generate, pack_batch, seed_for
+    dev = torch.device("cuda:0")
+    sizes = [1024] * 256 + [16384]
+    mats = [generate("normal", n, seed_for(5, n, b)) for b, n in enumerate(sizes)]
+    d_cat, e_cat, off = pack_batch(mats)
+    g_np = np.random.default_rng(5).standard_normal(d_cat.size)
+    bs = brdc.BatchSolver(sizes, device=dev)
+    d0 = torch.from_numpy(d_cat).to(dev)
+    e0 = torch.from_numpy(e_cat).to(dev)
+    g = torch.from_numpy(g_np).to(dev)
+    dt = d0.clone().requires_grad_(True)
+    et = e0.clone().requires_grad_(True)
+
+    def step():
+        dt.grad = None
+        et.grad = None
+        w = eigvalsh_tridiagonal_batched(dt, et, bs)
+        (w * g).sum().backward()
+        return w
+
+    def timed(fn, reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    for _ in range(warmup):
+        step()
+    ms_fb = timed(step, steps)
+    w = step().detach()
+    gd, ge = dt.grad.detach(), et.grad.detach()
+    wv, gdv, gev = w.cpu().numpy(), gd.cpu().numpy(), ge.cpu().numpy()
+    # forward alone (the batched solve) and backward alone (the gradient kernels)
+    wbuf = torch.empty_like(d0)
+
+    def fwd():
+        wbuf.copy_(d0)
+        bs.solve(wbuf, e0)
+    ms_f = timed(fwd, steps)
+    ms_b = timed(lambda: grad_batched(bs, d0, e0, w, g), steps)
+    # the plain single-solve GPU path: one forward solve per matrix
+    solvers = {n: brdc.Solver(n, device=dev) for n in set(sizes)}
+    dd = [torch.from_numpy(x).to(dev) for x, _ in mats]
+    ee = [torch.from_numpy(x).to(dev) for _, x in mats]
+    outs = [torch.empty_like(x) for x in dd]
+
+    def per_matrix():
+        for i in range(len(mats)):
+            outs[i].copy_(dd[i])
+            solvers[sizes[i]].solve(outs[i], ee[i])
+    per_matrix()
+    ms_pm = timed(per_matrix, max(1, steps // 2))
+    # accuracy: unit-eigenvector and Euler identities for every matrix, dense
+    # FP64 eigh gradients for two small ones
+    worst_id = 0.0
+    for b, (d, e) in enumerate(mats):
+        o, n = int(off[b]), sizes[b]
+        gb = g_np[o:o + n]
+        s = np.sum(np.abs(gb))
+        worst_id = max(worst_id, abs(gdv[o:o + n].sum() - gb.sum()) / s,
+                       abs(d @ gdv[o:o + n] + e @ gev[o:o + n - 1] - gb @ wv[o:o + n]) /
+                       (s * np.max(np.abs(wv[o:o + n]))))
+    worst_dense = 0.0
+    for b in (0, 200):
+        d, e = mats[b]
+        o, n = int(off[b]), sizes[b]
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        lam, V = np.linalg.eigh(T)
+        rd = (V * V) @ g_np[o:o + n]
+        re = 2.0 * (V[:-1] * V[1:]) @ g_np[o:o + n]
+        worst_dense = max(worst_dense, np.max(np.abs(gdv[o:o + n] - rd)),
+                          np.max(np.abs(gev[o:o + n - 1] - re)))
+    return {"workload": "256 tridiagonals at n=1024 + one at n=16384, D,E ~ N(0,1), upstream "
+                        "g ~ N(0,1); forward = one batched BR solve, backward = twisted-"
+                        "factorization eigenvector kernel (torch.autograd)",
+            "fwd_bwd_ms": round(ms_fb, 3), "fwd_ms": round(ms_f, 3), "bwd_ms": round(ms_b, 3),
+            "plain_per_matrix_fwd_ms": round(ms_pm, 3),
+            "worst_identity_rel_err": worst_id, "max_abs_err_vs_dense_eigh_grad": worst_dense,
+            "pass": bool(worst_id < 1e-10 and worst_dense < 1e-10)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +305,8 @@ def main():
                     help="launch kernels one by one instead of replaying a captured graph")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-workload section")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="skip the forward+backward section")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, three "
                          "times (calibrates the HBM byte counters of a profiler run)")
@@ -396,6 +491,8 @@ def main():
     if not args.no_batched:
         line["batched"] = bench_batched(max(3, args.steps // 2), max(1, args.warmup),
                                         want_cpu=not args.no_cpu_baseline)
+    if not args.no_grad:
+        line["differentiated"] = bench_differentiated(max(3, args.steps // 2), max(1, args.warmup))
     print(json.dumps(line), flush=True)
 
 

@@ -416,32 +416,28 @@ struct brdc_batch_plan {
 namespace brdc {
 namespace {
 
-// Gradient scratch: two per-wave pivot arrays of rows_max x 64 doubles for
-// each of `waves` waves, then the per-task partial sums.
-constexpr size_t kGradScratchCap = (size_t)4 << 30;  // bound on the pivot arrays
-
+// Gradient scratch: per wave three checkpoint arrays of (segments + 1) x 64
+// doubles, then the per-task partial sums.
 struct GradShape {
     int waves;
-    long long slot;  // doubles per pivot array per wave
-    size_t a, b, partial, total;
+    long long slot;  // doubles of checkpoints per wave
+    size_t cp, partial, total;
 };
 
 GradShape grad_shape(int ntask, int rows_max, long long partial_doubles) {
     GradShape G{};
-    G.slot = (long long)std::max(rows_max, 1) * kWave;
-    const size_t per_wave = 2 * (size_t)G.slot * sizeof(double);
-    int cap = (int)std::max<size_t>(kGradWaves, kGradScratchCap / per_wave);
-    cap = std::min(cap, 4096) / kGradWaves * kGradWaves;
+    const long long segs = (std::max(rows_max, 1) + kSeg - 1) / kSeg;
+    G.slot = 3 * (segs + 1) * kWave;
+    // one wave per task up to a few resident waves per SIMD
     G.waves = std::max(kGradWaves, std::min((ntask + kGradWaves - 1) / kGradWaves * kGradWaves,
-                                            cap));
+                                            4096));
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t at = off;
         off = align_up(off + bytes);
         return at;
     };
-    G.a = take((size_t)G.waves * G.slot * sizeof(double));
-    G.b = take((size_t)G.waves * G.slot * sizeof(double));
+    G.cp = take((size_t)G.waves * G.slot * sizeof(double));
     G.partial = take((size_t)std::max<long long>(partial_doubles, 1) * sizeof(double));
     G.total = off;
     return G;
@@ -452,14 +448,13 @@ int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const 
                  const double* e, const double* w, const double* g, double* dd, double* de,
                  void* scratch, hipStream_t st) {
     const GradShape G = grad_shape(ntask, rows_max, partial_doubles);
-    double* A = at<double>(scratch, G.a);
-    double* B = at<double>(scratch, G.b);
+    double* cp = at<double>(scratch, G.cp);
     double* part = at<double>(scratch, G.partial);
     const int n_one = moff ? 0 : rows;
     if (ntask > 0) {
         hipLaunchKernelGGL(k_grad_vectors, dim3(G.waves / kGradWaves), dim3(kWave * kGradWaves),
-                           0, st, nmat, moff, tprefix, poff, ntask, n_one, d, e, w, g, A, B,
-                           G.slot, part);
+                           0, st, nmat, moff, tprefix, poff, ntask, n_one, d, e, w, g, cp, G.slot,
+                           part);
         BRDC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
                            moff, tprefix, poff, n_one, (const double*)part, dd, de);

@@ -24,7 +24,9 @@
 
 namespace brdc {
 
-constexpr int kGradWaves = 4;  // waves per block of the vector kernel
+constexpr int kGradWaves = 4;   // waves per block of the vector kernel
+constexpr int kSeg = 16;        // rows per checkpoint segment
+constexpr int kWin = 256;       // rows of D and E staged in LDS per wave
 
 // Task t of a batch: matrix b with tprefix[b] <= t < tprefix[b+1], first
 // eigenvalue 64 * (t - tprefix[b]).  One matrix: tprefix == nullptr.
@@ -43,20 +45,57 @@ __device__ __forceinline__ double pivot_guard(double x, double pivmin) {
     return (fabs(x) < pivmin) ? -pivmin : x;
 }
 
-// slot: doubles per scratch array per wave (rows_max * 64).  partial: per
-// task 2 * n_b doubles at poff[b] + (t - tprefix[b]) * 2 * n_b.
+// Reciprocal on the factorizations' dependent chains: v_rcp_f64 and one
+// Newton step (relative error below 2^-50).  Each step of the stationary and
+// progressive factorizations is then exact for data perturbed by a few ulp,
+// which moves the recomputed eigenvector by O(n eps / gap) as rounding does.
+__device__ __forceinline__ double rcp_chain(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// Sum over the wave's 64 lanes of each of 32 values, scattered: returns in
+// lane L the total of value L/2 (lanes L and L^1 agree).  Halving on lane bits
+// 5..1 costs 16 + 8 + 4 + 2 + 1 exchanges, plus one for bit 0.
+__device__ __forceinline__ double reduce_scatter32(double* v, int lane) {
+#pragma unroll
+    for (int h = 16, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool hi = (lane & bit) != 0;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+            const double keep = hi ? v[h + i] : v[i];
+            const double send = hi ? v[i] : v[h + i];
+            v[i] = keep + __shfl_xor(send, bit, kWave);
+        }
+    }
+    return v[0] + __shfl_xor(v[0], 1, kWave);
+}
+
+// One task: 64 eigenvalues of one matrix, one per lane.  The pivots are not
+// stored: pass F keeps the stationary pivot D+ at every segment start (every
+// kSeg rows), pass B the progressive pivot D- likewise, and every later pass
+// recomputes a segment's pivots in registers from its checkpoints.  Per lane
+// the scratch holds three checkpoint rows per segment (D+, D-, z), so memory
+// traffic is O(n / kSeg) per eigenvalue and the passes are bound by their
+// dependent recurrences, not by memory.  All lanes walk the same segment at
+// the same time (per-lane differences are selects), so D and E arrive as
+// scalar loads and the last pass can sum each row over the wave's lanes.
+//   cp: per wave 3 arrays of (segments + 1) x 64 doubles: cpF[s] = D+ and
+//       cpB[s] = D- at row s*kSeg, zc[s] = z at row s*kSeg.
+//   partial: per task 2 * n_b doubles at poff[b] + (t - tprefix[b]) * 2 * n_b.
 __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     int nmat, const int* __restrict__ moff, const int* __restrict__ tprefix,
     const long long* __restrict__ poff, int ntask, int n_one, const double* __restrict__ d,
     const double* __restrict__ e, const double* __restrict__ w, const double* __restrict__ g,
-    double* __restrict__ bufA, double* __restrict__ bufB, long long slot,
-    double* __restrict__ partial) {
+    double* __restrict__ cpbuf, long long slot, double* __restrict__ partial) {
+    __shared__ double stage[kGradWaves][2][kWin];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = uniform((int)(blockIdx.x * kGradWaves + threadIdx.x / kWave));
+    const int wv = uniform((int)(threadIdx.x / kWave));
+    const int wave = uniform((int)(blockIdx.x * kGradWaves + wv));
     const int nwave = (int)gridDim.x * kGradWaves;
-    double* __restrict__ A = bufA + (long long)wave * slot;
-    double* __restrict__ B = bufB + (long long)wave * slot;
     const double safmin = 0x1p-1022;
+    double* sD = stage[wv][0];
+    double* sE = stage[wv][1];
     for (int t = wave; t < ntask; t += nwave) {
         const int b = uniform(grad_task_matrix(tprefix, nmat, t));
         const int off = moff ? uniform(moff[b]) : 0;
@@ -66,7 +105,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         double* __restrict__ part = partial + po + (long long)(t - t0) * 2 * n;
         const int i = (t - t0) * kWave + lane;
         const bool active = i < n;
-        const CDoubles D = as_const(d + off), Ev = as_const(e + off);
+        const CDoubles D = as_const(d + off);
         const double mu = w[off + (active ? i : 0)];
         const double gi = active ? g[off + i] : 0.0;
         // pivmin = safmin * max(1, max E^2) (DLARRD / DLAR1V)
@@ -81,72 +120,199 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             }
             continue;
         }
-        // forward stationary factorization: A[k] = D+_k
-        double dp = pivot_guard(D[0] - mu, pivmin);
-        A[lane] = dp;
-        for (int k = 0; k < n - 1; k++) {
-            const double ek = Ev[k];
-            const double l = ek / dp;
-            dp = pivot_guard((D[k + 1] - mu) - l * ek, pivmin);
-            A[(long long)(k + 1) * kWave + lane] = dp;
-        }
-        // backward factorization: B[k] = u_k; twist index r = argmin |gamma_k|
-        double dm = pivot_guard(D[n - 1] - mu, pivmin);
-        double gbest = fabs(A[(long long)(n - 1) * kWave + lane] + dm - (D[n - 1] - mu));
-        int r = n - 1;
-        for (int k = n - 2; k >= 0; k--) {
-            const double ek = Ev[k];
-            const double u = ek / dm;
-            B[(long long)k * kWave + lane] = u;
-            const double dk = D[k] - mu;
-            dm = pivot_guard(dk - u * ek, pivmin);
-            const double gam = fabs(A[(long long)k * kWave + lane] + dm - dk);
-            if (gam < gbest) {
-                gbest = gam;
-                r = k;
+        const int S = (n + kSeg - 1) / kSeg;  // segments; only the last may be short
+        double* __restrict__ cpF = cpbuf + (long long)wave * slot;
+        double* __restrict__ cpB = cpF + (long long)(S + 1) * kWave;
+        double* __restrict__ zc = cpB + (long long)(S + 1) * kWave;
+        auto at = [&](int s) { return (long long)s * kWave + lane; };
+        // D and E reach the segments through a window of kWin rows in this
+        // wave's LDS (refilled with coalesced loads when a segment leaves it;
+        // one wave's LDS operations complete in order, so no barrier).  A
+        // segment reads D of rows lo..lo+16 and E of rows lo-1..lo+15, clamped
+        // to the matrix (rows outside it only feed values that are never used).
+        int wbase = -(1 << 30);
+        auto window = [&](int lo, bool up) {
+            if (lo - 1 >= wbase && lo + kSeg + 1 <= wbase + kWin) return;
+            wbase = up ? lo - 1 : lo + kSeg + 1 - kWin;
+            for (int m = lane; m < kWin; m += kWave) {
+                const int row = min(max(wbase + m, 0), n - 1);
+                sD[m] = d[off + row];
+                sE[m] = e[off + min(row, n - 2)];
             }
-        }
-        // eigenvector: below r into B[k] (holding z_{k+1}), above r into A[k]
-        double nrm = 1.0, z = 1.0;
-        for (int k = 0; k < n - 1; k++) {
-            if (k >= r) {
-                const long long at = (long long)k * kWave + lane;
-                z = -B[at] * z;
-                B[at] = z;
-                nrm += z * z;
-            }
-        }
-        z = 1.0;
-        for (int k = n - 2; k >= 0; k--) {
-            if (k < r) {
-                const long long at = (long long)k * kWave + lane;
-                z = -(Ev[k] / A[at]) * z;
-                A[at] = z;
-                nrm += z * z;
-            }
-        }
-        // contributions g_i z_k^2 / |z|^2 and 2 g_i z_k z_{k+1} / |z|^2, summed
-        // over the wave's eigenvalues
-        const double s = active ? gi / nrm : 0.0;
-        auto zk_at = [&](int k) -> double {
-            if (k < r) return A[(long long)k * kWave + lane];
-            if (k == r) return 1.0;
-            return B[(long long)(k - 1) * kWave + lane];
         };
-        double zc = zk_at(0);
-        for (int k = 0; k < n; k++) {
-            const double zn = (k + 1 < n) ? zk_at(k + 1) : 0.0;
-            double cd = s * zc * zc;
-            double ce = 2.0 * s * zc * zn;
-            for (int o = 32; o > 0; o >>= 1) {
-                cd += __shfl_xor(cd, o, kWave);
-                ce += __shfl_xor(ce, o, kWave);
+        double dv[kSeg + 1], ev[kSeg], eprev;
+        auto load_seg = [&](int lo, bool up) {
+            window(lo, up);
+            const int o = lo - wbase;
+#pragma unroll
+            for (int j = 0; j <= kSeg; j++) dv[j] = sD[o + j];
+#pragma unroll
+            for (int j = 0; j < kSeg; j++) ev[j] = sE[o + j];
+            eprev = sE[o - 1];
+        };
+        auto step = [&](double ek, double x, double dk) {  // dk - ek^2 / x, guarded
+            return pivot_guard(fma(-ek * ek, rcp_chain(x), dk - mu), pivmin);
+        };
+        // P[j] = D+ of row lo+j from D+_lo
+        double P[kSeg], M[kSeg];
+        auto seg_dp = [&](double x) {
+#pragma unroll
+            for (int j = 0; j < kSeg; j++) {
+                P[j] = x;
+                x = step(ev[j], x, dv[j + 1]);
             }
-            if (lane == 0) {
-                part[k] = cd;
-                part[n + k] = ce;
+        };
+        // M[j] = D- of row lo+j: the top segment starts at D_{n-1} - mu, the
+        // others from D- of row lo+16 (xhi)
+        auto seg_dm = [&](int len, bool top, double xhi) {
+            double x = xhi;
+#pragma unroll
+            for (int j = kSeg - 1; j >= 0; j--) {
+                const double rec = step(ev[j], x, dv[j]);
+                const double start = top ? pivot_guard(dv[j] - mu, pivmin) : rec;
+                x = (j > len - 1) ? x : ((j == len - 1) ? start : rec);
+                M[j] = x;
             }
-            zc = zn;
+        };
+        // pass F: D+ chain over all rows, checkpoint at segment starts
+        {
+            double x = pivot_guard(D[0] - mu, pivmin);
+            for (int s = 0; s < S; s++) {
+                const int lo = s * kSeg;
+                load_seg(lo, true);
+                cpF[at(s)] = x;
+                seg_dp(x);
+                // D+ of row lo+16 continues the chain (P holds rows lo..lo+15)
+                x = step(ev[kSeg - 1], P[kSeg - 1], dv[kSeg]);
+            }
+        }
+        // pass B: D- chain downward; twist index r = argmin |gamma_k|
+        double gbest = HUGE_VAL;
+        int r = n - 1;
+        {
+            double xhi = 0.0;
+            double cpn = cpF[at(S - 1)];
+            for (int s = S - 1; s >= 0; s--) {
+                const int lo = s * kSeg, len = min(kSeg, n - lo);
+                const double cpc = cpn;
+                if (s > 0) cpn = cpF[at(s - 1)];  // next segment's checkpoint in flight
+                load_seg(lo, false);
+                seg_dp(cpc);
+                seg_dm(len, s == S - 1, xhi);
+#pragma unroll
+                for (int j = kSeg - 1; j >= 0; j--) {
+                    const double gam = fabs(P[j] + M[j] - (dv[j] - mu));
+                    const bool better = (j < len) && gam < gbest;
+                    gbest = better ? gam : gbest;
+                    r = better ? lo + j : r;
+                }
+                cpB[at(s)] = M[0];
+                xhi = M[0];
+            }
+        }
+        // pass Z: z with z_r = 1, |z|^2, and zc[s] = z at each segment start
+        double nrm = 1.0;
+        {
+            // rows below r: z_k = -(E_k / D+_k) z_{k+1}, segments from the top down
+            double z = 1.0;
+            double cpn = cpF[at(S - 1)];
+            for (int s = S - 1; s >= 0; s--) {
+                const int lo = s * kSeg, len = min(kSeg, n - lo);
+                const double cpc = cpn;
+                if (s > 0) cpn = cpF[at(s - 1)];
+                load_seg(lo, false);
+                seg_dp(cpc);
+#pragma unroll
+                for (int j = kSeg - 1; j >= 0; j--) {
+                    const int k = lo + j;
+                    const double zn = -(ev[j] * rcp_chain(P[j])) * z;
+                    z = (j >= len || k >= r) ? 1.0 : zn;
+                    nrm += (j < len && k < r) ? zn * zn : 0.0;
+                }
+                if (lo <= r) zc[at(s)] = z;
+            }
+            // rows above r: z_k = -(E_{k-1} / D-_k) z_{k-1}, segments upward
+            z = 1.0;
+            double cbn = (S > 1) ? cpB[at(1)] : 0.0;
+            for (int s = 0; s < S; s++) {
+                const int lo = s * kSeg, len = min(kSeg, n - lo);
+                const double cbc = cbn;
+                if (s + 2 < S) cbn = cpB[at(s + 2)];
+                load_seg(lo, true);
+                seg_dm(len, s == S - 1, cbc);
+#pragma unroll
+                for (int j = 0; j < kSeg; j++) {
+                    const int k = lo + j;
+                    const double ek = (j == 0) ? eprev : ev[j - 1];
+                    const double zn = -(ek * rcp_chain(M[j])) * z;
+                    const bool on = (j < len) && k > r;
+                    z = (k <= r) ? 1.0 : (j < len ? zn : z);
+                    nrm += on ? zn * zn : 0.0;
+                    if (j == 0 && lo > r) zc[at(s)] = z;
+                }
+            }
+        }
+        // pass C: contributions g_i z_k^2 / |z|^2 and 2 g_i z_k z_{k+1} / |z|^2,
+        // summed over the wave's eigenvalues, one segment at a time
+        const double sc = active ? gi / nrm : 0.0;
+        // checkpoints of the next two segments are in flight (F: D+, B: D- of
+        // the row after the segment, Z: z at the segment start)
+        auto ldF = [&](int q) { return (q < S) ? cpF[at(q)] : 0.0; };
+        auto ldB = [&](int q) { return (q < S) ? cpB[at(q)] : 0.0; };
+        auto ldZ = [&](int q) { return (q < S) ? zc[at(q)] : 0.0; };
+        double f0 = ldF(0), f1 = ldF(1), b0 = ldB(1), b1 = ldB(2), z0 = ldZ(0), z1 = ldZ(1),
+               z2 = ldZ(2);
+        for (int s = 0; s < S; s++) {
+            const int lo = s * kSeg, len = min(kSeg, n - lo);
+            const double cF = f0, cB = b0, zlo = z0, zhi = z1;  // zhi: z of row lo+16
+            f0 = f1;
+            b0 = b1;
+            z0 = z1;
+            z1 = z2;
+            if (s + 2 < S) {
+                f1 = ldF(s + 2);
+                b1 = ldB(s + 3);
+                z2 = ldZ(s + 3);
+            }
+            load_seg(lo, true);
+            seg_dp(cF);
+            seg_dm(len, s == S - 1, cB);
+            // below r: downward from z_{lo+16} (or from z_r = 1 inside the segment)
+            double x = zhi;
+#pragma unroll
+            for (int j = kSeg - 1; j >= 0; j--) {
+                const int k = lo + j;
+                x = (j >= len || k >= r) ? 1.0 : -(ev[j] * rcp_chain(P[j])) * x;
+                P[j] = x;
+            }
+            // at and above r: upward from z_lo (or from z_r = 1)
+            x = zlo;
+#pragma unroll
+            for (int j = 0; j < kSeg; j++) {
+                const int k = lo + j;
+                if (j > 0) x = (k <= r) ? 1.0 : -(ev[j - 1] * rcp_chain(M[j])) * x;
+                else x = (k <= r) ? 1.0 : zlo;
+                M[j] = (k < r) ? P[j] : x;  // z of row lo+j
+            }
+            // v[j] = dD contribution of row lo+j, v[16+j] = dE contribution
+            double v[2 * kSeg];
+#pragma unroll
+            for (int j = 0; j < kSeg; j++) {
+                const double zn = (j + 1 < kSeg) ? M[j + 1] : zhi;
+                const double znext = (j + 1 < len) ? zn : ((j + 1 == len) ? zhi : 0.0);
+                v[j] = sc * M[j] * M[j];
+                v[kSeg + j] = 2.0 * sc * M[j] * znext;
+            }
+            // reduce-scatter over the wave: at each halving the lane keeps the
+            // half of its values its lane bit selects and adds the partner's
+            // copy of it; after five halvings lane L holds (partly) value L/2,
+            // and a last exchange with lane L^1 completes it
+            const double total = reduce_scatter32(v, lane);
+            if ((lane & 1) == 0) {
+                const int idx = lane >> 1;
+                const int row = idx & (kSeg - 1);
+                if (row < len) part[(idx < kSeg ? 0 : n) + lo + row] = total;
+            }
         }
     }
 }
