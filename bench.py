@@ -293,6 +293,43 @@ def bench_differentiated(steps, warmup):
             "pass": bool(worst_id < 1e-10 and worst_dense < 1e-10)}
 
 
+def bench_vendor(n=4096):
+    """rocSOLVER STERF and STEDC (values only) on the same GPU for the
+    N=4096 normal matrix: a reported vendor baseline.  Both are sequential
+    QL/QR-style on this path and take seconds, so only this size is timed."""
+    import torch
+    import brdc
+    from brdc.comparators import rocsolver_eigvals
+    from brdc.generators import generate, seed_for
+    dev = torch.device("cuda:0")
+    d, e = generate("normal", n, seed_for(1, n))
+    dt = torch.from_numpy(d).to(dev)
+    et = torch.from_numpy(e).to(dev)
+    ours, info = brdc.dstebr_device(dt, et)
+    rocsolver_eigvals(dt[:64].contiguous(), et[:63].contiguous(), "sterf")  # library warm-up
+    torch.cuda.synchronize()
+    gs = brdc.GraphSolve(brdc.Solver(n, device=dev), dt, et)
+    for _ in range(3):
+        gs.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        gs.run()
+    torch.cuda.synchronize()
+    br_ms = (time.perf_counter() - t0) * 1e2
+    rec = {"workload": f"normal N={n} (the reference's smoke configuration), same GPU",
+           "br_ms": round(br_ms, 4)}
+    for m in ("sterf", "stedc"):
+        t0 = time.perf_counter()
+        w, inf = rocsolver_eigvals(dt, et, m)
+        torch.cuda.synchronize()
+        rec[f"rocsolver_{m}_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        rec[f"rocsolver_{m}_info"] = int(inf.item())
+        rec[f"max_rel_diff_br_vs_rocsolver_{m}"] = float((w - ours).abs().max() /
+                                                          ours.abs().max())
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +344,8 @@ def main():
                     help="skip the batched-workload section")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the forward+backward section")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the rocSOLVER comparison (about 20 s)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, three "
                          "times (calibrates the HBM byte counters of a profiler run)")
@@ -493,6 +532,8 @@ def main():
                                         want_cpu=not args.no_cpu_baseline)
     if not args.no_grad:
         line["differentiated"] = bench_differentiated(max(3, args.steps // 2), max(1, args.warmup))
+    if not args.no_vendor:
+        line["vendor_baseline"] = bench_vendor()
     print(json.dumps(line), flush=True)
 
 

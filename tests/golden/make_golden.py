@@ -8,7 +8,8 @@ library family the reference links, cpu/README.md):
     vectors (the reference calls DLAED4 at dlaed4_br.f:59 and dlaed9_br.f:87-127)
 
 Inputs come from brdc.generators, so the fixtures also pin the generators.
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [reduced]
+        (``reduced``: only the DSTERF fixture of reduced_dense_49152.de.bin)
 """
 from __future__ import annotations
 
@@ -105,7 +106,23 @@ def secular_cases(rng):
     return cases
 
 
+def reduced_dense_fixture():
+    """DSTERF eigenvalues of the reference's shipped input
+    data/inputs/reduced_dense_49152.de.bin (kept here as
+    reduced_dense_49152.de.bin: the compact tridiagonal of a reduced dense
+    matrix, int64 n then D then E)."""
+    from scipy.linalg.lapack import dsterf
+    from brdc.generators import read_de_bin
+    d, e = read_de_bin(os.path.join(HERE, "reduced_dense_49152.de.bin"))
+    w, info = dsterf(d.copy(), e.copy())[:2]
+    assert info == 0
+    np.savez_compressed(os.path.join(HERE, "reduced_dense_49152_dsterf.npz"), w=w)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "reduced":
+        reduced_dense_fixture()
+        return
     L = openblas()
     from scipy.linalg.lapack import dsterf
 

@@ -173,3 +173,46 @@ def test_graph_replay_matches_eager(gpu):
         x = torch.randn(1000, device=dev).sum()  # unrelated eager work
         torch.cuda.synchronize()
         assert torch.isfinite(x)
+
+
+def test_matches_rocsolver_on_the_same_gpu(gpu):
+    """rocSOLVER's STERF and STEDC (values only) as an independent GPU
+    reference at a size they finish quickly."""
+    from brdc.comparators import rocsolver_eigvals
+    torch = gpu
+    dev = torch.device("cuda:0")
+    for kind in ("normal", "toeplitz"):
+        d, e = generate(kind, 400, seed_for(1, 400))
+        w, info, _ = solve(torch, d, e)
+        dt, et = torch.from_numpy(d).to(dev), torch.from_numpy(e).to(dev)
+        for m in ("sterf", "stedc"):
+            ref, rinfo = rocsolver_eigvals(dt, et, m)
+            torch.cuda.synchronize()
+            assert int(rinfo.item()) == 0 and info == 0
+            assert rel_err(w, ref.cpu().numpy()) <= TOL
+
+
+def test_shipped_reduced_dense_input(gpu):
+    """The reference's shipped input (data/inputs/reduced_dense_49152.de.bin:
+    a reduced dense matrix; root merge K ~ 40000) against its DSTERF
+    fixture."""
+    from brdc.generators import read_de_bin
+    d, e = read_de_bin(os.path.join(GOLD, "reduced_dense_49152.de.bin"))
+    ref = np.load(os.path.join(GOLD, "reduced_dense_49152_dsterf.npz"))["w"]
+    w, info, _ = solve(gpu, d, e)
+    assert info == 0 and rel_err(w, ref) <= TOL
+    scale = np.max(np.abs(w))
+    n = d.shape[0]
+    assert abs(w.sum() - d.sum()) <= 1e-12 * scale * n
+
+
+def test_harness_gpu_columns(gpu):
+    import io
+    from brdc import harness
+    out = io.StringIO()
+    rows = harness.main(["--sizes=300", "--data=normal,toeplitz", "--iters=3", "--warmup=1",
+                         "--run-rocsolver=1", "--tridiag-diagnostics=1"], out=out)
+    for r in rows:
+        assert r["self_info"] == 0 and r["self_err"] <= TOL and r["rslv_err"] <= TOL
+        assert r["self_ms"] > 0 and r["rslv_ms"] > 0
+    assert "evals/root" in out.getvalue()
