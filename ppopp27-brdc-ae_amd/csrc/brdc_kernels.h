@@ -12,8 +12,8 @@
 //     r*   per root: lambda, tau, near1, near2 (+ int org)          [secular]
 //     w    product-form update vector                               [wprod]
 //     qo*  new boundary rows of the roots                           [rows]
-//   counts: kCountStride ints per merge (K, G, T, secular evaluations,
-//           root-claim counter)
+//   counts: kCountStride ints per merge (K, G, T, secular evaluations; the
+//           rest spare)
 // Every kernel derives its merge geometry from (n, depth, merge index), so a
 // level is a fixed launch sequence with no host round trip.
 #pragma once
@@ -777,7 +777,6 @@ __global__ __launch_bounds__(T) void k_deflate(
         counts[kCountStride * merge + 1] = G;
         counts[kCountStride * merge + 2] = Tn;
         counts[kCountStride * merge + 3] = 0;
-        counts[kCountStride * merge + 4] = 0;  // root-claim counter of the streamed secular kernel
     }
 }
 
@@ -986,9 +985,9 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
 // The waves' partial sums meet in LDS; wave 0 alone holds the roots' state
 // machines, advances them and publishes the next pass's evaluation points,
 // so a pass costs two barriers and one state-machine step.  A root group
-// whose root has converged takes the next unclaimed root of the merge
-// (per-merge atomic counter), so no lane idles while the merge has roots
-// left.  L is chosen per merge so that each root position works through
+// whose root has converged takes the next root of the workgroup's own
+// contiguous run of the merge's roots, so no lane idles while the run has
+// roots left.  L is chosen per run so that each root position works through
 // several roots (the last root's passes are a small tail of the total) while
 // each lane still streams a long run of poles per pass.
 //
@@ -1101,13 +1100,12 @@ __device__ __forceinline__ int wave_max(int x) {
     return x;
 }
 
-// Lanes per root of the streamed kernel for a merge of K poles at a level of
-// `merges` merges served by `groups` workgroups of W waves: enough lanes that
-// each of the groups * 64 / L root positions gets about 4 or more roots (the
-// merge's siblings assumed alike), no more than keeps 128 poles per lane per
-// pass, at most 16.
-__device__ __forceinline__ int stream_lanes(int K, int merges, int groups, int W) {
-    const long long want = (256LL * groups + (long long)K * merges - 1) / ((long long)K * merges);
+// Lanes per root of the streamed kernel for a workgroup's run of `roots`
+// roots of a K-pole merge: enough lanes that each of the 64 / L root
+// positions works through about 4 or more roots, no more than keeps 128 poles
+// per lane per pass, at most 16.
+__device__ __forceinline__ int stream_lanes(int K, int roots, int W) {
+    const int want = (256 + roots - 1) / max(roots, 1);
     int L = 1;
     while (L < 16 && L < want && (long long)K >= 2LL * W * 128 * L) L <<= 1;
     return L;
@@ -1127,10 +1125,30 @@ void k_secular_stream(
     __shared__ int pass_live;
     const int wave = uniform(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     double2* slot = stage[wave];
-    for (int mi = 0; mi < merges; mi++) {
-        const int merge = (int)((blockIdx.x + (unsigned)mi) % (unsigned)merges);
+    // Static split of the level's streamed roots: a root of a K-pole merge
+    // weighs K (its passes each stream K poles); workgroup b owns the roots
+    // whose weight interval starts in [W b / G, W (b+1) / G), a contiguous
+    // run of each merge, so its live roots -- and the exclusion window of a
+    // pass -- stay close together.
+    long long wsum = 0;
+    for (int m = 0; m < merges; m++) {
+        const long long K = uniform(counts[kCountStride * m]);
+        if (K >= kstream) wsum += K * K;
+    }
+    const long long wa = wsum * blockIdx.x / gridDim.x;
+    const long long wb = wsum * (blockIdx.x + 1) / gridDim.x;
+    long long wbase = 0;
+    for (int merge = 0; merge < merges; merge++) {
         const int K = uniform(counts[kCountStride * merge]);
         if (K < kstream) continue;
+        const long long w0 = wbase;
+        wbase += (long long)K * K;
+        // roots r with w0 + r K in [wa, wb)
+        auto first_root = [&](long long x) {
+            return (int)min<long long>(K, x <= w0 ? 0 : (x - w0 + K - 1) / K);
+        };
+        const int r0 = first_root(wa), r1 = first_root(wb);
+        if (r0 >= r1) continue;
         const Item it = g.item(merge);
         const int s = it.s, sz = it.sz;
         const FailSite fs = fail_site(info, moff, g.n, it);
@@ -1138,28 +1156,26 @@ void k_secular_stream(
         const double2* __restrict__ dz = adz + s;
         const PoleView E{dz};
         const int q0 = (int)(((long long)K * wave) / W), q1 = (int)(((long long)K * (wave + 1)) / W);
-        const int L = uniform(stream_lanes(K, merges, (int)gridDim.x, W));
+        const int L = uniform(stream_lanes(K, r1 - r0, W));
         const int c = lane & (L - 1), lead = lane - c;  // lane c of the root at lanes lead..
         // wave 0 owns the roots: state machine, claims, results
         SecularSolve<PoleView> st;
         int i = -1;  // this lane's root, -1 when it needs one
         bool drained = false;
+        int next = r0;  // next unclaimed root of this workgroup's run
         int iters = 0;
         for (;;) {
             if (wave == 0) {
                 if (!drained && __any(i < 0)) {
                     // one claim per root group (its lane c = 0), numbered by rank
                     const unsigned long long need = __ballot(i < 0 && c == 0);
-                    int base = 0;
-                    if (lane == 0)
-                        base = atomicAdd(&counts_rw[kCountStride * merge + 4], (int)__popcll(need));
-                    base = __shfl(base, 0, kWave);
-                    const int f = base + (int)__popcll(need & ((1ull << lead) - 1ull));
-                    if (i < 0 && f < K) {
+                    const int f = next + (int)__popcll(need & ((1ull << lead) - 1ull));
+                    next += (int)__popcll(need);
+                    if (i < 0 && f < r1) {
                         i = f;
                         st.init(E, K, i, rho);
                     }
-                    drained = __any(i < 0 && f >= K);  // the counter passed K
+                    drained = __any(i < 0 && f >= r1);  // the run is handed out
                 }
                 const bool active = (i >= 0);
                 pass_org[lane] = active ? st.ev_org : 0.0;
@@ -1250,7 +1266,16 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     const CDoubles d = as_const(ad + s);
     const CPairs rst = as_const(rs + s);
     const double di = ad[s + i];
-    double prod = 1.0;
+    // the product of ratios as a ratio of products num / den * 2^ex, with the
+    // exponents moved out after every batch: no division per pair
+    double num = 1.0, den = 1.0;
+    int ex = 0;
+    auto renorm = [&]() {
+        const int en = __builtin_amdgcn_frexp_exp(num), ed = __builtin_amdgcn_frexp_exp(den);
+        num = __builtin_amdgcn_ldexp(num, -en);
+        den = __builtin_amdgcn_ldexp(den, -ed);
+        ex += en - ed;
+    };
     // roots whose near1/near2 apply to a pole of this group: j in [g0-1, g0+63]
     // (near1 at pole j, near2 at pole j+1) and the last root (poles K-2, K-1)
     const int wa = max(j0, g0 - 1), wb = min(j1, g0 + kGroup);
@@ -1259,9 +1284,11 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
         if (check) {  // j == i and the pole-adjacent roots lie in the window only
             if (i == j) dl = rn1[s + j];
             else if (i == j + 1) dl = rn2[s + j];
-            prod *= (j == i) ? dl : dl * rcp_nr(di - dj);
+            num *= dl;
+            den *= (j == i) ? 1.0 : di - dj;
         } else {
-            prod *= dl * rcp_nr(di - dj);
+            num *= dl;
+            den *= di - dj;
         }
     };
     auto range = [&](int ja, int jb, bool check) {
@@ -1276,8 +1303,12 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
             }
 #pragma unroll
             for (int b = 0; b < kBatch; b++) term(r[b], dj[b], j + b, check);
+            renorm();
         }
-        for (; j < jb; j++) term(rst[j], d[j], j, check);
+        for (; j < jb; j++) {
+            term(rst[j], d[j], j, check);
+            renorm();
+        }
     };
     const int jlast = (K - 1 >= j0 && K - 1 < j1) ? K - 1 : -1;
     const int jend = (jlast >= 0) ? K - 1 : j1;  // the last root is handled apart
@@ -1293,8 +1324,11 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
         double dl = (di - r.x) - r.y;
         if (i == K - 2) dl = rn1[s + jlast];
         else if (i == K - 1) dl = rn2[s + jlast];
-        prod *= (jlast == i) ? dl : dl * rcp_nr(di - ad[s + jlast]);
+        num *= dl;
+        den *= (jlast == i) ? 1.0 : di - ad[s + jlast];
+        renorm();
     }
+    double prod = __builtin_amdgcn_ldexp(num / den, ex);  // this wave's share of the product
     if (W > 1) {
         red[c][lane] = prod;
         __syncthreads();
@@ -1355,7 +1389,7 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
             if (i == a) dl = nr1;
             else if (i == a + 1) dl = nr2;
         }
-        const double x = p.y * rcp_nr(dl);
+        const double x = p.y * rcp_stream(dl);
         temp = fma(x, x, temp);
         acc1 = fma(q.x, x, acc1);
         acc2 = fma(q.y, x, acc2);

@@ -8,7 +8,7 @@
 namespace brdc {
 
 constexpr int kWave = 64;
-// ints per merge record: K, G, T, secular evaluations, root-claim counter
+// ints per merge record: K, G, T, secular evaluations (the rest spare)
 constexpr int kCountStride = 8;
 
 // lanes per root: about 4+ poles per lane per pass, and no more lanes than
