@@ -10,7 +10,12 @@ closed form for Toeplitz, the CPU oracle's eigenvalues from the cpu_baseline
 leg for the others, and the trace / Frobenius invariants for all).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 49152]
-       [--no-cpu-baseline] [--leaf L]
+       [--no-cpu-baseline] [--leaf L] [--eager] [--no-batched] [--no-grad]
+       [--no-vendor] [--no-compare] [--only batched|differentiated|vendor]
+
+Besides the main line's fields, the JSON line carries the batched workload
+(2048 matrices, solves/s), the differentiated workload (forward + backward
+ms) and rocSOLVER on the same GPU; --only runs one of those sections alone.
 """
 from __future__ import annotations
 
@@ -113,10 +118,42 @@ def copy_probe(mib):
                       "launches": 3}), flush=True)
 
 
-def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True):
+def per_matrix_comparison(mats, sizes, ws, ms, leaf):
+    """The plain per-matrix GPU path on the batch's matrices (one solve per
+    matrix, eager launches): time for the whole batch and the largest
+    difference from the batched results ws (every 64th matrix)."""
+    import torch
+    import brdc
+    dev = torch.device("cuda:0")
+    solvers = {n: brdc.Solver(n, device=dev, leaf_max=leaf) for n in set(int(x) for x in sizes)}
+    dd = [torch.from_numpy(d).to(dev) for d, _ in mats]
+    ee = [torch.from_numpy(e).to(dev) for _, e in mats]
+    outs = [torch.empty_like(x) for x in dd]
+    for i in range(min(32, len(mats))):
+        outs[i].copy_(dd[i])
+        solvers[int(sizes[i])].solve(outs[i], ee[i])
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for i in range(len(mats)):
+        outs[i].copy_(dd[i])
+        solvers[int(sizes[i])].solve(outs[i], ee[i])
+    b.record()
+    torch.cuda.synchronize()
+    per_matrix_ms = a.elapsed_time(b)
+    diff = max(rel_err(ws[i], outs[i].cpu().numpy()) for i in range(0, len(mats), 64))
+    return {"per_matrix_gpu_ms_per_batch": round(per_matrix_ms, 3),
+            "per_matrix_gpu_solves_per_s": round(len(mats) / (per_matrix_ms * 1e-3), 1),
+            "speedup_vs_per_matrix_gpu": round(per_matrix_ms / ms, 2),
+            "max_rel_diff_vs_per_matrix_gpu": diff}
+
+
+def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True, leaf=0, compare=True):
     """The batched workload (2048 tridiagonals, sizes log-uniform in [32, 4096],
-    D, E ~ N(0,1)) as one flattened merge forest, leaf bound autotuned;
-    reported beside the plain per-matrix GPU path and the CPU oracle."""
+    D, E ~ N(0,1)) as one flattened merge forest, leaf bound autotuned (or
+    fixed by leaf > 0); reported beside the plain per-matrix GPU path (unless
+    compare is false) and the CPU oracle."""
     import torch
     import brdc
     from brdc import autotune
@@ -127,7 +164,8 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True):
     d_dev = torch.from_numpy(d_cat).to(dev)
     e_dev = torch.from_numpy(e_cat).to(dev)
     t0 = time.perf_counter()
-    tune = autotune.tune_batched(d_dev, e_dev, sizes, reps=3)
+    tune = autotune.tune_batched(d_dev, e_dev, sizes, reps=3) if leaf <= 0 else \
+        {"leaf_max": leaf, "ms": {}}
     tune_s = time.perf_counter() - t0
     bs = brdc.BatchSolver(sizes, device=dev, leaf_max=tune["leaf_max"])
     gs = brdc.GraphSolve(bs, d_dev, e_dev)
@@ -146,36 +184,15 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True):
     infos = bs.info.cpu().numpy()[:bs.nmat]
     ws = bs.split(w)
     inv = max(invariants_err(ws[i], *mats[i]) for i in range(len(mats)))
-    # the plain per-matrix GPU path on the same matrices (one solve per matrix)
-    solvers = {}
-    dd = [torch.from_numpy(d).to(dev) for d, _ in mats]
-    ee = [torch.from_numpy(e).to(dev) for _, e in mats]
-    outs = [torch.empty_like(x) for x in dd]
-    for n in set(int(x) for x in sizes):
-        solvers[n] = brdc.Solver(n, device=dev, leaf_max=tune["leaf_max"])
-    for i in range(min(32, len(mats))):
-        outs[i].copy_(dd[i])
-        solvers[int(sizes[i])].solve(outs[i], ee[i])
-    torch.cuda.synchronize()
-    a.record()
-    for i in range(len(mats)):
-        outs[i].copy_(dd[i])
-        solvers[int(sizes[i])].solve(outs[i], ee[i])
-    b.record()
-    torch.cuda.synchronize()
-    per_matrix_ms = a.elapsed_time(b)
-    per_matrix_diff = max(rel_err(ws[i], outs[i].cpu().numpy()) for i in range(0, len(mats), 64))
     rec = {"workload": "2048 tridiagonals, n = round(exp(U(ln 32, ln 4096))), D,E ~ N(0,1), "
                        "one flattened merge forest per launch sequence (hipGraph replay)",
            "matrices": len(mats), "rows": int(off[-1]), "leaf_max": tune["leaf_max"],
            "autotune_ms_per_batch": {str(k): round(v, 4) for k, v in tune["ms"].items()},
            "autotune_s": round(tune_s, 2),
            "ms_per_batch": round(ms, 4), "solves_per_s": round(len(mats) / (ms * 1e-3), 1),
-           "per_matrix_gpu_ms_per_batch": round(per_matrix_ms, 3),
-           "per_matrix_gpu_solves_per_s": round(len(mats) / (per_matrix_ms * 1e-3), 1),
-           "speedup_vs_per_matrix_gpu": round(per_matrix_ms / ms, 2),
-           "info_nonzero": int(np.count_nonzero(infos)), "worst_invariants": inv,
-           "max_rel_diff_vs_per_matrix_gpu": per_matrix_diff}
+           "info_nonzero": int(np.count_nonzero(infos)), "worst_invariants": inv}
+    if compare:
+        rec.update(per_matrix_comparison(mats, sizes, ws, ms, tune["leaf_max"]))
     if want_cpu:
         import oracle
         if not os.path.exists(oracle.LIB_PATH):
@@ -195,15 +212,17 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True):
                                "sample": f"oracle BR restatement on the first {done} matrices "
                                          f"of the batch ({int(sum(sizes[:done]))} rows)"}
         rec["max_rel_err_vs_cpu_oracle"] = worst
-    rec["pass"] = bool(rec["info_nonzero"] == 0 and inv < 1e-10 and per_matrix_diff < 1e-10
+    rec["pass"] = bool(rec["info_nonzero"] == 0 and inv < 1e-10
+                       and rec.get("max_rel_diff_vs_per_matrix_gpu", 0.0) < 1e-10
                        and rec.get("max_rel_err_vs_cpu_oracle", 0.0) <= 1e-10)
     return rec
 
 
-def bench_differentiated(steps, warmup):
+def bench_differentiated(steps, warmup, compare=True):
     """The differentiated workload: 256 tridiagonals at n = 1024 plus one at
     n = 16384, D, E ~ N(0,1), upstream g ~ N(0,1); forward (one batched
-    solve) + hand-written backward through torch.autograd."""
+    solve) + hand-written backward through torch.autograd; beside it the
+    plain per-matrix forward (unless compare is false)."""
     import torch
     import brdc
     from brdc.autograd import eigvalsh_tridiagonal_batched, grad_batched
@@ -253,17 +272,19 @@ def bench_differentiated(steps, warmup):
     ms_f = timed(fwd, steps)
     ms_b = timed(lambda: grad_batched(bs, d0, e0, w, g), steps)
     # the plain single-solve GPU path: one forward solve per matrix
-    solvers = {n: brdc.Solver(n, device=dev) for n in set(sizes)}
-    dd = [torch.from_numpy(x).to(dev) for x, _ in mats]
-    ee = [torch.from_numpy(x).to(dev) for _, x in mats]
-    outs = [torch.empty_like(x) for x in dd]
+    ms_pm = None
+    if compare:
+        solvers = {n: brdc.Solver(n, device=dev) for n in set(sizes)}
+        dd = [torch.from_numpy(x).to(dev) for x, _ in mats]
+        ee = [torch.from_numpy(x).to(dev) for _, x in mats]
+        outs = [torch.empty_like(x) for x in dd]
 
-    def per_matrix():
-        for i in range(len(mats)):
-            outs[i].copy_(dd[i])
-            solvers[sizes[i]].solve(outs[i], ee[i])
-    per_matrix()
-    ms_pm = timed(per_matrix, max(1, steps // 2))
+        def per_matrix():
+            for i in range(len(mats)):
+                outs[i].copy_(dd[i])
+                solvers[sizes[i]].solve(outs[i], ee[i])
+        per_matrix()
+        ms_pm = round(timed(per_matrix, max(1, steps // 2)), 3)
     # accuracy: unit-eigenvector and Euler identities for every matrix, dense
     # FP64 eigh gradients for two small ones
     worst_id = 0.0
@@ -288,7 +309,7 @@ def bench_differentiated(steps, warmup):
                         "g ~ N(0,1); forward = one batched BR solve, backward = twisted-"
                         "factorization eigenvector kernel (torch.autograd)",
             "fwd_bwd_ms": round(ms_fb, 3), "fwd_ms": round(ms_f, 3), "bwd_ms": round(ms_b, 3),
-            "plain_per_matrix_fwd_ms": round(ms_pm, 3),
+            "plain_per_matrix_fwd_ms": ms_pm,
             "worst_identity_rel_err": worst_id, "max_abs_err_vs_dense_eigh_grad": worst_dense,
             "pass": bool(worst_id < 1e-10 and worst_dense < 1e-10)}
 
@@ -346,6 +367,12 @@ def main():
                     help="skip the forward+backward section")
     ap.add_argument("--no-vendor", action="store_true",
                     help="skip the rocSOLVER comparison (about 20 s)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the per-matrix GPU comparisons of the batched and "
+                         "differentiated sections")
+    ap.add_argument("--only", choices=("batched", "differentiated", "vendor"), default=None,
+                    help="run only this section and print its record as the JSON line "
+                         "(for profiling one workload at a time)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, three "
                          "times (calibrates the HBM byte counters of a profiler run)")
@@ -358,6 +385,15 @@ def main():
               "the multi-GPU layout of this path is a design note, not a measured run",
               file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        rec = {"batched": lambda: bench_batched(args.steps, args.warmup,
+                                                want_cpu=not args.no_cpu_baseline,
+                                                leaf=args.leaf, compare=not args.no_compare),
+               "differentiated": lambda: bench_differentiated(args.steps, args.warmup,
+                                                              compare=not args.no_compare),
+               "vendor": bench_vendor}[args.only]()
+        print(json.dumps({"section": args.only, **rec}), flush=True)
+        return
 
     import torch
     import brdc
@@ -529,9 +565,11 @@ def main():
         line["speedup_vs_cpu_baseline"] = round(cpu_rec["value"] / value, 2) if value > 0 else None
     if not args.no_batched:
         line["batched"] = bench_batched(max(3, args.steps // 2), max(1, args.warmup),
-                                        want_cpu=not args.no_cpu_baseline)
+                                        want_cpu=not args.no_cpu_baseline,
+                                        compare=not args.no_compare)
     if not args.no_grad:
-        line["differentiated"] = bench_differentiated(max(3, args.steps // 2), max(1, args.warmup))
+        line["differentiated"] = bench_differentiated(max(3, args.steps // 2), max(1, args.warmup),
+                                                      compare=not args.no_compare)
     if not args.no_vendor:
         line["vendor_baseline"] = bench_vendor()
     print(json.dumps(line), flush=True)

@@ -214,7 +214,8 @@ def profile_kernels(solver: Solver, problems, stream=None):
     Returns {"kernels": {class: {"ms", "launches", "avg_us"}}, "flops": {class:
     FP64 flops}, "bytes": {class: bytes}} summed over the problems, where the
     flop counts use each solve's own per-merge K and secular evaluation
-    counts: 7 flops per pole per evaluation (secular), 3 per pole pair
+    counts: 7 flops per pole per evaluation (secular for merges below
+    STREAM_MIN_K poles, secular_stream for the others), 3 per pole pair
     (product-form w, non-root merges), 9 per pole pair (boundary rows,
     non-root merges).
     """
@@ -223,7 +224,7 @@ def profile_kernels(solver: Solver, problems, stream=None):
     nc = len(_lib.KERNEL_CLASSES)
     tot_ms = np.zeros(nc)
     tot_n = np.zeros(nc, dtype=np.int64)
-    flops = {"secular": 0.0, "wprod": 0.0, "rows": 0.0}
+    flops = {"secular": 0.0, "secular_stream": 0.0, "wprod": 0.0, "rows": 0.0}
     nbytes = {k: 0.0 for k in _BYTES_PER_ROW}
     st = stream if stream is not None else torch.cuda.current_stream()
     for (w, d, e) in problems:
@@ -239,7 +240,10 @@ def profile_kernels(solver: Solver, problems, stream=None):
         tot_n += cnt
         stats = solver.merge_stats(st)
         k = stats["k"].astype(np.float64)
-        flops["secular"] += float(np.sum(7.0 * k * stats["evals"]))
+        sec = 7.0 * k * stats["evals"]
+        big = stats["k"] >= _lib.STREAM_MIN_K
+        flops["secular"] += float(np.sum(sec[~big]))
+        flops["secular_stream"] += float(np.sum(sec[big]))
         nonroot = (stats["level"] > 0) & (stats["k"] > 2)
         flops["wprod"] += float(np.sum(3.0 * k[nonroot] ** 2))
         flops["rows"] += float(np.sum(9.0 * k[nonroot] ** 2))

@@ -25,6 +25,10 @@ namespace brdc {
 namespace {
 
 constexpr int kDefaultLeaf = 16;
+// merges up to this many poles are staged in LDS (4 waves x 640 x 16 B =
+// 40 KB per block, so the LDS admits the 3 blocks per CU that the secular
+// kernel's register budget allows)
+constexpr int kSecStageK = 640;
 constexpr int kSecWaves = 4;       // waves per block of the secular kernel (independent waves)
 constexpr int kSecMaxWaves = 3072;  // secular grid cap: about 3 resident waves per SIMD
 constexpr int kStreamWaves = 8;     // waves per workgroup of the streamed secular kernel
@@ -154,7 +158,7 @@ T* at(void* ws, size_t off) {
 // MergeLevelTimer, gpu/src/tridiag_solver.cu:158-359): when enabled on the
 // calling host thread, every launch is bracketed by two events on its stream.
 enum KernelClass { kNorm, kPrepE, kLeaf, kPrepare, kDeflate, kSecular, kWprod, kRows, kFinalize,
-                   kNumClasses };
+                   kSecularStream, kNumClasses };
 struct Profiler {
     bool on = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
@@ -335,15 +339,18 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
             const int slots = secular_slots(std::min(maxsz, kStreamMinK - 1), merges);
             const int waves = std::min(merges * slots, kSecMaxWaves);
             const int blocks = (waves + kSecWaves - 1) / kSecWaves;
-            hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves), 0, st,
-                               g, merges, slots, streamed ? kStreamMinK : INT_MAX,
+            // LDS pole stage per wave: the level's largest merge, capped
+            const int stage_k = std::min(maxsz, kSecStageK);
+            const size_t lds = sizeof(double2) * kSecWaves * stage_k;
+            hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves), lds,
+                               st, g, merges, slots, streamed ? kStreamMinK : INT_MAX, stage_k,
                                (const double*)ew, (const int*)counts, (const double*)ad,
                                (const double*)az, (const double2*)adz, rlam, rs, rn1, rn2, counts,
                                S.moff, info);
         }
         BRDC_LAUNCH_CHECK();
         if (streamed) {
-            ProfScope ps(kSecular, st);
+            ProfScope ps(kSecularStream, st);
             const int groups = std::min(stream_groups(), (n + kWave - 1) / kWave);
             hipLaunchKernelGGL(k_secular_stream<kStreamWaves>, dim3(groups),
                                dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,

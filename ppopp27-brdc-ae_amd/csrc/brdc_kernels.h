@@ -871,12 +871,13 @@ __device__ __forceinline__ void secular_side(const double2* __restrict__ dz, int
     }
 }
 
-// One group of 64/L roots of one merge.  Returns the group's secular
+// One group of 64/L roots of one merge; dz is the merge's pole list (d_j,
+// z_j^2), in global memory or staged in LDS.  Returns the group's secular
 // iteration count (summed over the wave).
 __device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, int K, int L,
                                              int grp, double rho, const double* __restrict__ ad,
                                              const double* __restrict__ az,
-                                             const double2* __restrict__ adz,
+                                             const double2* __restrict__ dz,
                                              double* __restrict__ rlam, double2* __restrict__ rs,
                                              double* __restrict__ rn1,
                                              double* __restrict__ rn2) {
@@ -902,7 +903,6 @@ __device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, 
         }
         return (i == 0 && c == 0) ? 1 : 0;
     }
-    const double2* __restrict__ dz = adz + s;
     const PoleView E{dz};
     SecularSolve<PoleView> st;
     bool active = (i < K);
@@ -941,22 +941,34 @@ __device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, 
 }
 
 // All roots of the level's merges with K < kstream.  Work is split into
-// tasks (merge, slot), slot < slots; the task's wave solves root groups slot, slot + slots, ... of its
-// merge, so every group is covered whatever the slot count, and the grid is
-// capped, each wave walking tasks with a grid stride.
+// tasks (merge, slot), slot < slots; the task's wave solves root groups slot,
+// slot + slots, ... of its merge, so every group is covered whatever the slot
+// count, and the grid is capped, each wave walking tasks with a grid stride.  A merge of at most
+// stage_k poles is first copied into the wave's own stage_k-entry slice of
+// the dynamic LDS (WL * stage_k double2), so every pass and every pole
+// lookup of the root solver reads LDS; larger merges read global memory.
+// One wave's LDS operations complete in order, so the copy needs no barrier.
+// Three waves per SIMD (at most 168 VGPRs): the root solver's state machine
+// is latency-bound, and the occupancy hides its dependent FP64 chains better
+// than the few spills it costs.
 template <int WL>
-__global__ __launch_bounds__(kWave * WL) void k_secular(
-    Geo g, int merges, int slots, int kstream, const double* __restrict__ ew, const int* counts,
-    const double* __restrict__ ad, const double* __restrict__ az,
+__global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_secular(
+    Geo g, int merges, int slots, int kstream, int stage_k, const double* __restrict__ ew,
+    const int* counts, const double* __restrict__ ad, const double* __restrict__ az,
     const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
     double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw,
     const int* __restrict__ moff, int* info) {
+    extern __shared__ double2 sec_stage[];
     const int ntask = merges * slots;
     const int nwave = gridDim.x * WL;
     const int lane = threadIdx.x & (kWave - 1);
+    double2* const stage = sec_stage + (threadIdx.x / kWave) * stage_k;
     for (int task = uniform(blockIdx.x * WL + (int)(threadIdx.x / kWave)); task < ntask;
          task += nwave) {
-        const int merge = task / slots, slot = task - merge * slots;
+        // slot-major: the tasks a merge actually needs (its first slots)
+        // are contiguous across merges, so the grid stride spreads them
+        // evenly over the waves whatever the slot over-provisioning
+        const int slot = task / merges, merge = task - slot * merges;
         const int K = uniform(counts[kCountStride * merge]);
         const int L = secular_lanes(K, merges);
         const int rpw = kWave / L;  // roots per wave
@@ -966,8 +978,18 @@ __global__ __launch_bounds__(kWave * WL) void k_secular(
         const FailSite fs = fail_site(info, moff, g.n, it);
         const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
         int iters = 0;
-        for (int grp = slot; grp * rpw < K; grp += slots)
-            iters += secular_group(fs, s, sz, K, L, grp, rho, ad, az, adz, rlam, rs, rn1, rn2);
+        if (K <= stage_k) {
+            for (int j = lane; j < K; j += kWave) stage[j] = adz[s + j];
+            __builtin_amdgcn_wave_barrier();
+            for (int grp = slot; grp * rpw < K; grp += slots)
+                iters += secular_group(fs, s, sz, K, L, grp, rho, ad, az, stage, rlam, rs, rn1,
+                                       rn2);
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            for (int grp = slot; grp * rpw < K; grp += slots)
+                iters += secular_group(fs, s, sz, K, L, grp, rho, ad, az, adz + s, rlam, rs, rn1,
+                                       rn2);
+        }
         for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
         if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
     }

@@ -13,7 +13,7 @@ constexpr int kCountStride = 8;
 
 // lanes per root: about 4+ poles per lane per pass, and no more lanes than
 // the level needs to fill the machine (~kTargetLanes lanes in flight)
-constexpr int kTargetLanes = 1 << 19;
+constexpr int kTargetLanes = 1 << 18;
 __host__ __device__ inline int secular_lanes(int K, int merges) {
     int L = 1;
     while (L < kWave && 4 * 2 * L <= K && (long long)merges * K * (2 * L) <= kTargetLanes) L <<= 1;

@@ -26,6 +26,21 @@ def test_library_exports_every_header_symbol():
         assert hasattr(L, name), name
 
 
+def test_profile_classes_match_the_library():
+    """The Python mirror of the kernel classes and of the streamed-merge
+    threshold agrees with the header and the HIP source."""
+    hdr = open(os.path.join(ROOT, "include", "brdc.h")).read()
+    assert int(re.search(r"#define BRDC_KERNEL_CLASSES (\d+)", hdr).group(1)) == \
+        len(_lib.KERNEL_CLASSES)
+    src = open(os.path.join(ROOT, "ppopp27-brdc-ae_amd", "csrc", "brdc.hip")).read()
+    enum = re.search(r"enum KernelClass \{([^}]*)\}", src).group(1)
+    names = [x.strip() for x in enum.split(",") if x.strip() and x.strip() != "kNumClasses"]
+    camel = ["k" + "".join(p.capitalize() for p in c.split("_")) for c in _lib.KERNEL_CLASSES]
+    assert names == camel
+    assert int(re.search(r"constexpr int kStreamMinK = (\d+);", src).group(1)) == \
+        _lib.STREAM_MIN_K
+
+
 def test_status_strings():
     L = _lib.load()
     assert L.brdc_status_string(0) == b"success"
