@@ -19,6 +19,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "brdc_launch.h"
 #include "brdc_secular.h"
@@ -1059,11 +1060,13 @@ __device__ __forceinline__ void staged_range(const double2* __restrict__ dz, dou
 }
 
 // Poles [ja, jb) that lie wholly on one side of every root; this lane (c of
-// its root's L lanes) takes poles ja + c, ja + c + L, ...
+// its root's L lanes) takes poles ja + c, ja + c + L, ...  L is a template
+// parameter so the batch's LDS reads take immediate offsets.
+template <int L>
 __device__ __forceinline__ void stream_side(const double2* __restrict__ dz, double2* slot, int ja,
-                                            int jb, int c, int L, double org, double tau,
+                                            int jb, int c, double org, double tau,
                                             StreamAcc& a) {
-    const double dL = (double)L;
+    constexpr double dL = (double)L;
     staged_range(dz, slot, ja, jb, [&](int c0, int nb) {
         const int nfull = nb / (kStreamBatch * L);  // batches every lane has in full
         int o = c;
@@ -1074,12 +1077,20 @@ __device__ __forceinline__ void stream_side(const double2* __restrict__ dz, doub
             // J gets sum_b (j + b L) t_b = j * T + L * sum_b b t_b
             double T = 0.0, B = 0.0;
 #pragma unroll
-            for (int b = 0; b < kStreamBatch; b++) {
-                const double r = rcp_stream((p[b].x - org) - tau);
-                const double t = p[b].y * r;
-                T += t;
-                a.D = fma(t, r, a.D);
-                if (b > 0) B = fma((double)b, t, B);
+            for (int b = 0; b < kStreamBatch; b += 2) {
+                // one reciprocal for two poles: 1/x = y/(xy), 1/y = x/(xy)
+                // (deltas of the scaled problem lie far inside the range
+                // where xy neither overflows nor underflows)
+                const double x = (p[b].x - org) - tau, y = (p[b + 1].x - org) - tau;
+                const double rxy = rcp_stream(x * y);
+                const double r0 = y * rxy, r1 = x * rxy;
+                const double t0 = p[b].y * r0, t1 = p[b + 1].y * r1;
+                T += t0;
+                T += t1;
+                a.D = fma(t0, r0, a.D);
+                a.D = fma(t1, r1, a.D);
+                if (b > 0) B = fma((double)b, t0, B);
+                B = fma((double)(b + 1), t1, B);
             }
             a.S += T;
             a.J = fma((double)(c0 + o), T, a.J);
@@ -1213,9 +1224,20 @@ void k_secular_stream(
             const int exlo = pass_exlo[lane], nex = pass_nex[lane];
             const int wlo = uniform(wave_min(exlo)), whi = uniform(wave_max(nex > 0 ? exlo + nex : 0));
             StreamAcc lo = {0.0, 0.0, 0.0}, hi = {0.0, 0.0, 0.0};
-            stream_side(dz, slot, q0, min(q1, wlo), c, L, org, tau, lo);
-            stream_window(dz, slot, max(q0, wlo), min(q1, whi), c, L, org, tau, exlo, nex, lo, hi);
-            stream_side(dz, slot, max(q0, whi), q1, c, L, org, tau, hi);
+            auto pass = [&](auto lanes) {
+                constexpr int LL = decltype(lanes)::value;
+                stream_side<LL>(dz, slot, q0, min(q1, wlo), c, org, tau, lo);
+                stream_window(dz, slot, max(q0, wlo), min(q1, whi), c, LL, org, tau, exlo, nex,
+                              lo, hi);
+                stream_side<LL>(dz, slot, max(q0, whi), q1, c, org, tau, hi);
+            };
+            switch (L) {  // stream_lanes returns 1, 2, 4, 8 or 16
+                case 1: pass(std::integral_constant<int, 1>{}); break;
+                case 2: pass(std::integral_constant<int, 2>{}); break;
+                case 4: pass(std::integral_constant<int, 4>{}); break;
+                case 8: pass(std::integral_constant<int, 8>{}); break;
+                default: pass(std::integral_constant<int, 16>{}); break;
+            }
             double acc[6] = {lo.S, lo.D, lo.J, hi.S, hi.D, hi.J};
             for (int o = 1; o < L; o <<= 1) {
 #pragma unroll
@@ -1405,13 +1427,15 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
     const CPairs P = as_const(pw + s);
     const CPairs Q = as_const(aq + s);
     double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
-    auto term = [&](double2 p, double2 q, int i, bool check) {
+    auto delta = [&](double2 p, int i, bool check) {
         double dl = (p.x - dorg) - tau;
         if (check) {
             if (i == a) dl = nr1;
             else if (i == a + 1) dl = nr2;
         }
-        const double x = p.y * rcp_stream(dl);
+        return dl;
+    };
+    auto accumulate = [&](double x, double2 q) {
         temp = fma(x, x, temp);
         acc1 = fma(q.x, x, acc1);
         acc2 = fma(q.y, x, acc2);
@@ -1426,9 +1450,15 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
                 q[b] = Q[i + b];
             }
 #pragma unroll
-            for (int b = 0; b < kBatch; b++) term(p[b], q[b], i + b, check);
+            for (int b = 0; b < kBatch; b += 2) {
+                // one reciprocal for two poles: 1/u = v/(uv), 1/v = u/(uv)
+                const double u = delta(p[b], i + b, check), v = delta(p[b + 1], i + b + 1, check);
+                const double ruv = rcp_stream(u * v);
+                accumulate(p[b].y * (v * ruv), q[b]);
+                accumulate(p[b + 1].y * (u * ruv), q[b + 1]);
+            }
         }
-        for (; i < ib; i++) term(P[i], Q[i], i, check);
+        for (; i < ib; i++) accumulate(P[i].y * rcp_stream(delta(P[i], i, check)), Q[i]);
     };
     // poles that may be pole-adjacent to a root of this group: [g0, g0+64]
     // and the last root's K-2, K-1
