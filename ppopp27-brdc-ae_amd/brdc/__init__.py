@@ -154,9 +154,12 @@ class BatchSolver:
                    "batch upload")
 
     def __del__(self):
+        # at interpreter exit the module globals may already be cleared; the
+        # process is ending then and the host-side plan goes with it
         plan = getattr(self, "_plan", None)
-        if plan and _lib._lib is not None:
-            _lib._lib.brdc_batch_plan_destroy(plan)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
+        if plan and lib is not None:
+            lib.brdc_batch_plan_destroy(plan)
             self._plan = None
 
     def stats(self) -> dict:

@@ -54,12 +54,15 @@ __device__ __forceinline__ double rcp_chain(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
-// Sum over the wave's 64 lanes of each of 32 values, scattered: returns in
-// lane L the total of value L/2 (lanes L and L^1 agree).  Halving on lane bits
-// 5..1 costs 16 + 8 + 4 + 2 + 1 exchanges, plus one for bit 0.
-__device__ __forceinline__ double reduce_scatter32(double* v, int lane) {
+// Sum over the wave's 64 lanes of each of NV values (NV a power of two up to
+// 64), scattered: returns in lane L the total of value L / (64 / NV) (the
+// 64 / NV lanes that share a value agree).  Halving on lane bits 5, 4, ...
+// costs NV/2 + NV/4 + ... + 1 exchanges, plus log2(64 / NV) for the lanes
+// that share a value.
+template <int NV>
+__device__ __forceinline__ double reduce_scatter(double* v, int lane) {
 #pragma unroll
-    for (int h = 16, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+    for (int h = NV / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
         const bool hi = (lane & bit) != 0;
 #pragma unroll
         for (int i = 0; i < h; i++) {
@@ -68,7 +71,10 @@ __device__ __forceinline__ double reduce_scatter32(double* v, int lane) {
             v[i] = keep + __shfl_xor(send, bit, kWave);
         }
     }
-    return v[0] + __shfl_xor(v[0], 1, kWave);
+    double t = v[0];
+#pragma unroll
+    for (int bit = kWave / NV / 2; bit >= 1; bit >>= 1) t += __shfl_xor(t, bit, kWave);
+    return t;
 }
 
 // One task: 64 eigenvalues of one matrix, one per lane.  The pivots are not
@@ -305,11 +311,11 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             }
             // reduce-scatter over the wave: at each halving the lane keeps the
             // half of its values its lane bit selects and adds the partner's
-            // copy of it; after five halvings lane L holds (partly) value L/2,
-            // and a last exchange with lane L^1 completes it
-            const double total = reduce_scatter32(v, lane);
-            if ((lane & 1) == 0) {
-                const int idx = lane >> 1;
+            // copy of it; afterwards lane L holds value L / kShare
+            constexpr int kShare = kWave / (2 * kSeg);  // lanes holding each total
+            const double total = reduce_scatter<2 * kSeg>(v, lane);
+            if ((lane & (kShare - 1)) == 0) {
+                const int idx = lane / kShare;
                 const int row = idx & (kSeg - 1);
                 if (row < len) part[(idx < kSeg ? 0 : n) + lo + row] = total;
             }

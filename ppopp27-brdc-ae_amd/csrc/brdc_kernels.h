@@ -28,7 +28,6 @@
 namespace brdc {
 
 constexpr int kMaxLeaf = 64;
-constexpr int kJacobiSweeps = 40;
 constexpr int kSpecCap = 256;     // speculative Givens-run walk bound
 constexpr int kThreads = 256;     // block size of the element/root-parallel kernels
 
@@ -232,148 +231,130 @@ __global__ void k_prep_e(int n, const int* __restrict__ moff, int nmat,
 }
 
 // ---------------------------------------------------------------------------
-// Leaf eigensolve: one wavefront per leaf (<= MAXM rows) holding the dense
-// leaf in LDS.  Cyclic Jacobi in round-robin (tournament) order: each round
-// picks m/2 disjoint (p, q) pairs, computes their rotations, and applies
-// A <- J^T A J as independent 2x2 block updates -- block (a, b) of the pair
-// structure becomes R_a^T A[P_a, P_b] R_b -- so a round is one LDS pass with
-// no intermediate barrier.  Only the first and last rows of the accumulated
-// rotation product are kept (the role of DSTEQR_BOUNDARY_BR,
-// dlaed0_br.f:354-371).  The rank-one cut corrections (dlaed0_br.f:95-103)
-// are applied to the leaf's end rows on load.
+// Leaf eigensolve, one lane per leaf (the DSTEQR_BOUNDARY_BR role,
+// dlaed0_br.f:354-371): implicit QL with the Wilkinson shift on the leaf's
+// tridiagonal (Bowdler, Martin, Reinsch and Wilkinson), with each rotation
+// applied only to the first and last rows of the eigenvector matrix, so a
+// leaf costs O(m^2) flops.  The rank-one cut corrections (dlaed0_br.f:95-103)
+// are applied to the leaf's end rows on load.  A lane's d, e and two rows
+// live in LDS, element k of lane l at [k][l]: no lane touches another's
+// entries, so there is no barrier.  Eigenvalues leave sorted ascending with
+// their rows.
+__device__ __forceinline__ double lapy2(double x, double y) {
+    const double xa = fabs(x), ya = fabs(y);
+    const double w = fmax(xa, ya), z = fmin(xa, ya);
+    if (z == 0.0 || w > 1.79e308) return w;
+    const double q = z * rcp_nr(w);
+    return w * sqrt(fma(q, q, 1.0));
+}
+
+// lanes (leaves) per block: four LDS vectors of MAXM doubles per lane
 template <int MAXM>
-__global__ __launch_bounds__(kWave) void k_leaf(Geo g, const int* __restrict__ moff,
-                                                const double* __restrict__ d_in,
-                                                const double* __restrict__ ew,
-                                                const double* __restrict__ scal, double* out_final,
-                                                double* out_d, double* out_lo, double* out_hi,
-                                                int* info, int* info_user) {
-    constexpr int LD = MAXM + 1;
-    __shared__ double A[MAXM * LD];
-    __shared__ double V0[MAXM], V1[MAXM];
-    __shared__ double RC[MAXM / 2], RS[MAXM / 2];
-    __shared__ int RP[MAXM / 2], RQ[MAXM / 2];
-    __shared__ int changed;
-    const Item it = g.item(blockIdx.x);
+constexpr int leaf_lanes() { return MAXM <= 32 ? kWave : kWave / 2; }
+
+template <int MAXM>
+__global__ __launch_bounds__(leaf_lanes<MAXM>()) void k_leaf(
+    Geo g, int nleaves, const int* __restrict__ moff, const double* __restrict__ d_in,
+    const double* __restrict__ ew, const double* __restrict__ scal, double* out_final,
+    double* out_d, double* out_lo, double* out_hi, int* info, int* info_user) {
+    constexpr int LP = leaf_lanes<MAXM>();
+    __shared__ double sd[MAXM][LP], se[MAXM][LP], s0[MAXM][LP], s1[MAXM][LP];
+    const int lane = threadIdx.x;
+    const int leaf = blockIdx.x * LP + lane;
+    if (leaf >= nleaves) return;
+    const Item it = g.item(leaf);
     const int s = it.s, m = it.sz, n = g.n;
     const bool single = (it.flags & kItemRoot) != 0;  // the leaf is its whole matrix
-    const int lane = threadIdx.x;
     const double scale = scal[2 * it.mat];
-    for (int idx = lane; idx < MAXM * LD; idx += kWave) A[idx] = 0.0;
-    __syncthreads();
-    if (lane < m) {
-        double di = d_in[s + lane] * scale;
-        if (lane == 0 && s > 0) di -= fabs(ew[s - 1]);
-        if (lane == m - 1 && s + m < n) di -= fabs(ew[s + m - 1]);
-        A[lane * LD + lane] = di;
-        if (lane < m - 1) {
-            const double el = ew[s + lane];
-            A[lane * LD + lane + 1] = el;
-            A[(lane + 1) * LD + lane] = el;
-        }
-        V0[lane] = (lane == 0) ? 1.0 : 0.0;
-        V1[lane] = (lane == m - 1) ? 1.0 : 0.0;
+    for (int k = 0; k < m; k++) {
+        double dk = d_in[s + k] * scale;
+        if (k == 0 && s > 0) dk -= fabs(ew[s - 1]);
+        if (k == m - 1 && s + m < n) dk -= fabs(ew[s + m - 1]);
+        sd[k][lane] = dk;
+        se[k][lane] = (k < m - 1) ? ew[s + k] : 0.0;
+        s0[k][lane] = (k == 0) ? 1.0 : 0.0;
+        s1[k][lane] = (k == m - 1) ? 1.0 : 0.0;
     }
-    __syncthreads();
-    // an odd m gets a zero dummy row/column m; its pairs never rotate
-    const int mm = m + (m & 1);
-    const int npair = mm / 2;
-    const int nblk = npair * npair;
-    const float inv_npair = 1.0f / (float)npair;
-    bool converged = (m <= 1);
-    for (int sweep = 0; sweep < kJacobiSweeps && !converged; sweep++) {
-        if (lane == 0) changed = 0;
-        __syncthreads();
-        for (int r = 0; r < mm - 1; r++) {
-            if (lane < npair) {
-                // round-robin slots: slot 0 fixed, slots 1..mm-1 rotate by r
-                int xa = lane - 1 + r, xb = mm - 2 - lane + r;
-                if (xa >= mm - 1) xa -= mm - 1;
-                if (xb >= mm - 1) xb -= mm - 1;
-                const int a = (lane == 0) ? 0 : xa + 1;
-                const int b = xb + 1;
-                const int p = min(a, b), q = max(a, b);
-                double c = 1.0, sn = 0.0;
-                if (q < m) {
-                    const double apq = A[p * LD + q];
-                    const double app = A[p * LD + p], aqq = A[q * LD + q];
-                    if (apq != 0.0 && fabs(apq) > kEps * sqrt(fabs(app)) * sqrt(fabs(aqq))) {
-                        const double theta = (aqq - app) * rcp_nr(2.0 * apq);
-                        double t;
-                        if (fabs(theta) > 1e150)
-                            t = 0.5 * rcp_nr(theta);
-                        else
-                            t = copysign(rcp_nr(fabs(theta) + sqrt(theta * theta + 1.0)), theta);
-                        c = rcp_nr(sqrt(t * t + 1.0));
-                        sn = t * c;
-                        changed = 1;
-                    }
+    bool failed = false;
+    const int maxit = 30 * m;
+    int iter = 0;
+    for (int l = 0; l < m && !failed; l++) {
+        for (;;) {
+            // the first negligible off-diagonal at or after l
+            int mm = l;
+            for (; mm < m - 1; mm++) {
+                const double dd = fabs(sd[mm][lane]) + fabs(sd[mm + 1][lane]);
+                if (fabs(se[mm][lane]) <= kEps * dd) break;
+            }
+            if (mm == l) break;  // d[l] is an eigenvalue
+            if (++iter > maxit) {
+                failed = true;
+                break;
+            }
+            const double dl = sd[l][lane], el = se[l][lane];
+            double gg = (sd[l + 1][lane] - dl) * rcp_nr(2.0 * el);
+            double r = lapy2(gg, 1.0);
+            gg = sd[mm][lane] - dl + el * rcp_nr(gg + copysign(r, gg));
+            double sn = 1.0, c = 1.0, p = 0.0;
+            bool underflow = false;
+            // d[k+1] and the two rows' entries k+1 ride in registers from one
+            // step to the next; each step loads entry k and stores entry k+1
+            double dk1 = sd[mm][lane], x1 = s0[mm][lane], y1 = s1[mm][lane];
+            for (int k = mm - 1; k >= l; k--) {
+                const double ek = se[k][lane], dk = sd[k][lane];
+                const double x0 = s0[k][lane], y0 = s1[k][lane];
+                const double f = sn * ek, b = c * ek;
+                r = lapy2(f, gg);
+                se[k + 1][lane] = r;
+                if (r == 0.0) {  // recover from underflow
+                    sd[k + 1][lane] = dk1 - p;
+                    s0[k + 1][lane] = x1;
+                    s1[k + 1][lane] = y1;
+                    se[mm][lane] = 0.0;
+                    underflow = true;
+                    break;
                 }
-                RC[lane] = c;
-                RS[lane] = sn;
-                RP[lane] = p;
-                RQ[lane] = q;
+                const double rr = rcp_nr(r);
+                sn = f * rr;
+                c = gg * rr;
+                gg = dk1 - p;
+                r = (dk - gg) * sn + 2.0 * c * b;
+                p = sn * r;
+                sd[k + 1][lane] = gg + p;
+                gg = c * r - b;
+                // the two kept rows of the eigenvector matrix
+                s0[k + 1][lane] = sn * x0 + c * x1;
+                x1 = c * x0 - sn * x1;
+                s1[k + 1][lane] = sn * y0 + c * y1;
+                y1 = c * y0 - sn * y1;
+                dk1 = dk;
             }
-            __syncthreads();
-            for (int blk = lane; blk < nblk; blk += kWave) {
-                const int a = (int)(((float)blk + 0.5f) * inv_npair), b = blk - a * npair;
-                const double sa = RS[a], sb = RS[b];
-                if (sa == 0.0 && sb == 0.0) continue;
-                const double ca = RC[a], cb = RC[b];
-                const int pa = RP[a], qa = RQ[a], pb = RP[b], qb = RQ[b];
-                const double x00 = A[pa * LD + pb], x01 = A[pa * LD + qb];
-                const double x10 = A[qa * LD + pb], x11 = A[qa * LD + qb];
-                // rows (J_a^T), then columns (J_b)
-                const double y00 = ca * x00 - sa * x10, y01 = ca * x01 - sa * x11;
-                const double y10 = sa * x00 + ca * x10, y11 = sa * x01 + ca * x11;
-                double z00 = cb * y00 - sb * y01, z01 = sb * y00 + cb * y01;
-                double z10 = cb * y10 - sb * y11, z11 = sb * y10 + cb * y11;
-                if (a == b) {  // the annihilated pair
-                    z01 = 0.0;
-                    z10 = 0.0;
-                }
-                A[pa * LD + pb] = z00;
-                A[pa * LD + qb] = z01;
-                A[qa * LD + pb] = z10;
-                A[qa * LD + qb] = z11;
-            }
-            if (lane < npair && RS[lane] != 0.0) {  // first/last rows of V <- V J
-                const double c = RC[lane], sn = RS[lane];
-                const int p = RP[lane], q = RQ[lane];
-                double x = V0[p], y = V0[q];
-                V0[p] = c * x - sn * y;
-                V0[q] = sn * x + c * y;
-                x = V1[p];
-                y = V1[q];
-                V1[p] = c * x - sn * y;
-                V1[q] = sn * x + c * y;
-            }
-            __syncthreads();
+            if (underflow) continue;
+            s0[l][lane] = x1;
+            s1[l][lane] = y1;
+            sd[l][lane] = dk1 - p;
+            se[l][lane] = gg;
+            se[mm][lane] = 0.0;
         }
-        converged = (changed == 0);
-        __syncthreads();
     }
-    if (!converged && lane == 0) report_item_failure(fail_site(info, moff, n, it), m);
-    // sort ascending, carrying the two rows
-    __shared__ double L[MAXM];
-    if (lane < m) L[lane] = A[lane * LD + lane];
-    __syncthreads();
-    if (lane < m) {
-        const double x = L[lane];
+    if (failed) report_item_failure(fail_site(info, moff, n, it), m);
+    // ascending order by rank (ties by position), carrying the two rows
+    for (int a = 0; a < m; a++) {
+        const double x = sd[a][lane];
         int rank = 0;
-        for (int k = 0; k < m; k++) {
-            const double y = L[k];
-            rank += (y < x) || (y == x && k < lane);
+        for (int b = 0; b < m; b++) {
+            const double y = sd[b][lane];
+            rank += (y < x) || (y == x && b < a);
         }
         if (single) {
             out_final[s + rank] = x * scal[2 * it.mat + 1];
         } else {
             out_d[s + rank] = x;
-            out_lo[s + rank] = V0[lane];
-            out_hi[s + rank] = V1[lane];
+            out_lo[s + rank] = s0[a][lane];
+            out_hi[s + rank] = s1[a][lane];
         }
     }
-    if (single && lane == 0 && info_user) *info_user = *info;  // one matrix only
+    if (single && info_user) *info_user = *info;  // a one-leaf solve: one matrix
 }
 
 // ---------------------------------------------------------------------------
