@@ -30,6 +30,11 @@ constexpr int kDefaultLeaf = 16;
 // kernel's register budget allows)
 constexpr int kSecStageK = 640;
 constexpr int kSecWaves = 4;       // waves per block of the secular kernel (independent waves)
+// levels whose merges are all at most this large, and at least this many,
+// run the packed secular mode (roots of several merges per wave); below
+// that count the per-merge groups with LDS-staged poles are faster
+constexpr int kPackMaxSize = 256;
+constexpr int kPackMinMerges = 4096;
 constexpr int kSecMaxWaves = 3072;  // secular grid cap: about 3 resident waves per SIMD
 constexpr int kStreamWaves = 8;     // waves per workgroup of the streamed secular kernel
 constexpr int kStreamMinK = 2048;   // merges with at least this many poles are streamed
@@ -335,7 +340,19 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
         // merges of K >= kStreamMinK poles go to the streamed kernel, the
         // rest to the lanes-per-root kernel
         const bool streamed = (maxsz >= kStreamMinK);
-        {
+        if (maxsz <= kPackMaxSize && merges >= kPackMinMerges) {
+            // many small merges: roots packed across merges (rend is free
+            // between deflations and holds the root offsets)
+            ProfScope ps(kSecular, st);
+            hipLaunchKernelGGL(k_root_scan, dim3(1), dim3(kScanThreads), 0, st, merges,
+                               streamed ? kStreamMinK : INT_MAX, (const int*)counts, rend);
+            BRDC_LAUNCH_CHECK();
+            const int blocks = kSecMaxWaves / kSecWaves;
+            hipLaunchKernelGGL(k_secular_packed<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves),
+                               0, st, g, merges, (const int*)rend, (const double*)ew,
+                               (const int*)counts, (const double*)ad, (const double*)az,
+                               (const double2*)adz, rlam, rs, rn1, rn2, counts, S.moff, info);
+        } else {
             ProfScope ps(kSecular, st);
             // wave slots per merge: the most any K it handles can need; the
             // grid is capped and walks (merge, slot) tasks

@@ -977,6 +977,148 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
     }
 }
 
+// Packed secular mode for levels of many small merges: the level's roots
+// (merges with K < kstream, in merge order) are numbered 0 .. total-1, root
+// i of merge m being roff[m] + i, and a wave takes 64/L consecutive roots
+// whatever merges they belong to, so no lane idles for a merge whose K is
+// not a multiple of the group.  k_root_scan forms roff (exclusive prefix of
+// the merges' K, roff[merges] = total) in one block.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_root_scan(int merges, int kstream,
+                                                             const int* __restrict__ counts,
+                                                             int* __restrict__ roff) {
+    __shared__ int part[kScanThreads];
+    const int t = threadIdx.x;
+    const int per = (merges + kScanThreads - 1) / kScanThreads;
+    const int m0 = min(merges, t * per), m1 = min(merges, m0 + per);
+    auto roots = [&](int m) {
+        const int K = counts[kCountStride * m];
+        return K < kstream ? K : 0;
+    };
+    int sum = 0;
+    for (int m = m0; m < m1; m++) sum += roots(m);
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {  // inclusive Hillis-Steele scan
+        const int v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int off = part[t] - sum;
+    for (int m = m0; m < m1; m++) {
+        roff[m] = off;
+        off += roots(m);
+    }
+    if (t == kScanThreads - 1) roff[merges] = part[t];
+}
+
+// All roots of the level's merges with K < kstream, packed (see k_root_scan):
+// L lanes per root from the level's mean K, each wave a contiguous run of
+// root groups.  A lane's merge is found by a walk over roff from the run's
+// first merge (found once by binary search).  Poles are read from global
+// memory (the L1 cache holds the few merges a group spans).
+template <int WL>
+__global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_secular_packed(
+    Geo g, int merges, const int* __restrict__ roff, const double* __restrict__ ew,
+    const int* __restrict__ counts, const double* __restrict__ ad,
+    const double* __restrict__ az, const double2* __restrict__ adz, double* __restrict__ rlam,
+    double2* __restrict__ rs, double* __restrict__ rn1, double* __restrict__ rn2,
+    int* counts_rw, const int* __restrict__ moff, int* info) {
+    const int total = uniform(roff[merges]);
+    if (total == 0) return;
+    const int L = secular_lanes((total + merges - 1) / merges, merges);
+    const int rpg = kWave / L;  // roots per group
+    const long long groups = (total + rpg - 1) / rpg;
+    const int nwave = gridDim.x * WL;
+    const int wave = uniform(blockIdx.x * WL + (int)(threadIdx.x / kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = lane & (L - 1);
+    const int g0 = (int)(groups * wave / nwave), g1 = (int)(groups * (wave + 1) / nwave);
+    if (g0 >= g1) return;
+    // the merge holding root g0 * rpg: the last m with roff[m] <= R
+    int mrun;
+    {
+        const int R = g0 * rpg;
+        int lo = 0, hi = merges;  // roff[lo] <= R < roff[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (roff[mid] <= R) lo = mid;
+            else hi = mid;
+        }
+        mrun = uniform(lo);
+    }
+    for (int grp = g0; grp < g1; grp++) {
+        const int R0 = grp * rpg;
+        while (roff[mrun + 1] <= R0) mrun++;  // wave-uniform walk
+        const int R = R0 + lane / L;
+        const bool has = R < total;
+        int m = mrun;
+        if (has)
+            while (roff[m + 1] <= R) m++;
+        const int K = has ? counts[kCountStride * m] : 0;
+        const int i = R - roff[m];
+        const Item it = g.item(m);
+        const int s = it.s, sz = it.sz;
+        const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
+        int iters = 0;
+        bool active = has && K >= 3;
+        if (has && K <= 2 && c == 0) {  // two-pole closed form / single pole
+            if (K == 1) {
+                const double z0 = az[s];
+                rlam[s] = ad[s] + rho * z0 * z0;
+                rs[s] = make_double2(ad[s], rho * z0 * z0);
+                rn1[s] = 1.0;
+                rn2[s] = 0.0;
+            } else {
+                double lam, v0, v1, tau;
+                secular_two(i, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
+                rlam[s + i] = lam;
+                rs[s + i] = make_double2(ad[s], tau);
+                rn1[s + i] = v0;
+                rn2[s + i] = v1;
+            }
+            iters = (i == 0) ? 1 : 0;
+        }
+        const double2* __restrict__ dz = adz + s;
+        const PoleView E{dz};
+        SecularSolve<PoleView> st;
+        st.init(E, active ? K : 3, active ? i : 0, rho);
+        for (;;) {
+            if (!__any(active)) break;
+            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (active) {
+                const int j2 = st.ev_exlo + st.ev_nex;
+                secular_side(dz, c, st.ev_exlo, L, st.ev_org, st.ev_tau, acc[0], acc[1],
+                             acc[2]);
+                secular_side(dz, j2 + ((c - j2) & (L - 1)), K, L, st.ev_org, st.ev_tau, acc[3],
+                             acc[4], acc[5]);
+            }
+            for (int o = 1; o < L; o <<= 1) {
+#pragma unroll
+                for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+            }
+            SecSums sums;
+            sums.S = acc[0] + acc[3];
+            sums.A = acc[3] - acc[0];
+            sums.dpsi = acc[1];
+            sums.dphi = acc[4];
+            sums.J = acc[2] + acc[5];
+            if (active && st.advance(E, sums)) active = false;
+        }
+        if (has && K >= 3 && c == 0) {
+            const RootState& r = st.out;
+            rlam[s + i] = r.lambda;
+            rs[s + i] = make_double2(dz[r.org].x, r.tau);
+            rn1[s + i] = r.near1;
+            rn2[s + i] = r.near2;
+            iters = r.iters;
+            if (r.info) report_item_failure(fail_site(info, moff, g.n, it), sz);
+        }
+        if (iters) atomicAdd(&counts_rw[kCountStride * m + 3], iters);
+    }
+}
+
 // Streamed secular solve for the merges with K >= kstream (the big merges
 // of the top levels, where the O(K^2) work sits).
 //
