@@ -277,13 +277,14 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
     double* shi[2] = {at<double>(ws, L.shi[0]), at<double>(ws, L.shi[1])};
     {
         ProfScope ps(kLeaf, st);
-        // one lane per leaf; the instance fits the largest leaf
-        auto kern = (S.leaf_rows <= 16) ? k_leaf<16> : (S.leaf_rows <= 32) ? k_leaf<32> : k_leaf<64>;
-        const int lanes = (S.leaf_rows <= 32) ? leaf_lanes<32>() : leaf_lanes<64>();
-        hipLaunchKernelGGL(kern, dim3((S.nleaves + lanes - 1) / lanes), dim3(lanes), 0, st,
-                           S.leaves, S.nleaves, S.moff, (const double*)d, (const double*)ew,
-                           (const double*)scal, d, sd[cur], slo[cur], shi[cur], info,
-                           S.levels.empty() ? info_user : nullptr);
+        // one lane per leaf, LDS sized by the largest leaf
+        const int maxm = std::max(S.leaf_rows, 1);
+        const int lanes = leaf_lanes(maxm);
+        hipLaunchKernelGGL(lanes == kWave ? k_leaf<kWave> : k_leaf<kWave / 2>,
+                           dim3((S.nleaves + lanes - 1) / lanes), dim3(lanes),
+                           leaf_lds_bytes(maxm), st, S.leaves, S.nleaves, maxm, S.moff,
+                           (const double*)d, (const double*)ew, (const double*)scal, d, sd[cur],
+                           slo[cur], shi[cur], info, S.levels.empty() ? info_user : nullptr);
     }
     BRDC_LAUNCH_CHECK();
     if (S.levels.empty()) return BRDC_STATUS_SUCCESS;

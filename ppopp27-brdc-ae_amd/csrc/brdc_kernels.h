@@ -248,20 +248,33 @@ __device__ __forceinline__ double lapy2(double x, double y) {
     return w * sqrt(fma(q, q, 1.0));
 }
 
-// lanes (leaves) per block: four LDS vectors of MAXM doubles per lane
-template <int MAXM>
-constexpr int leaf_lanes() { return MAXM <= 32 ? kWave : kWave / 2; }
+// Lanes (leaves) per block for leaves of at most maxm rows: each lane holds
+// four LDS vectors of maxm doubles, and a block stays within 64 KB.
+__host__ __device__ inline int leaf_lanes(int maxm) { return maxm <= 32 ? kWave : kWave / 2; }
+__host__ __device__ inline size_t leaf_lds_bytes(int maxm) {
+    return sizeof(double) * 4 * (size_t)maxm * leaf_lanes(maxm);
+}
 
-template <int MAXM>
-__global__ __launch_bounds__(leaf_lanes<MAXM>()) void k_leaf(
-    Geo g, int nleaves, const int* __restrict__ moff, const double* __restrict__ d_in,
+// maxm: the largest leaf's row count (sizes the dynamic LDS: the fewer bytes
+// per leaf, the more leaves are resident per CU); LP = leaf_lanes(maxm).
+template <int LP>
+__global__ __launch_bounds__(LP) void k_leaf(
+    Geo g, int nleaves, int maxm, const int* __restrict__ moff, const double* __restrict__ d_in,
     const double* __restrict__ ew, const double* __restrict__ scal, double* out_final,
     double* out_d, double* out_lo, double* out_hi, int* info, int* info_user) {
-    constexpr int LP = leaf_lanes<MAXM>();
-    __shared__ double sd[MAXM][LP], se[MAXM][LP], s0[MAXM][LP], s1[MAXM][LP];
+    extern __shared__ double leaf_lds[];
     const int lane = threadIdx.x;
     const int leaf = blockIdx.x * LP + lane;
     if (leaf >= nleaves) return;
+    // element k of this lane's vectors at [k * LP + lane]
+    double* const vd = leaf_lds + lane;
+    double* const ve = vd + (size_t)maxm * LP;
+    double* const v0 = ve + (size_t)maxm * LP;
+    double* const v1 = v0 + (size_t)maxm * LP;
+#define sd_(k) vd[(k) * LP]
+#define se_(k) ve[(k) * LP]
+#define s0_(k) v0[(k) * LP]
+#define s1_(k) v1[(k) * LP]
     const Item it = g.item(leaf);
     const int s = it.s, m = it.sz, n = g.n;
     const bool single = (it.flags & kItemRoot) != 0;  // the leaf is its whole matrix
@@ -270,10 +283,10 @@ __global__ __launch_bounds__(leaf_lanes<MAXM>()) void k_leaf(
         double dk = d_in[s + k] * scale;
         if (k == 0 && s > 0) dk -= fabs(ew[s - 1]);
         if (k == m - 1 && s + m < n) dk -= fabs(ew[s + m - 1]);
-        sd[k][lane] = dk;
-        se[k][lane] = (k < m - 1) ? ew[s + k] : 0.0;
-        s0[k][lane] = (k == 0) ? 1.0 : 0.0;
-        s1[k][lane] = (k == m - 1) ? 1.0 : 0.0;
+        sd_(k) = dk;
+        se_(k) = (k < m - 1) ? ew[s + k] : 0.0;
+        s0_(k) = (k == 0) ? 1.0 : 0.0;
+        s1_(k) = (k == m - 1) ? 1.0 : 0.0;
     }
     bool failed = false;
     const int maxit = 30 * m;
@@ -283,34 +296,34 @@ __global__ __launch_bounds__(leaf_lanes<MAXM>()) void k_leaf(
             // the first negligible off-diagonal at or after l
             int mm = l;
             for (; mm < m - 1; mm++) {
-                const double dd = fabs(sd[mm][lane]) + fabs(sd[mm + 1][lane]);
-                if (fabs(se[mm][lane]) <= kEps * dd) break;
+                const double dd = fabs(sd_(mm)) + fabs(sd_(mm + 1));
+                if (fabs(se_(mm)) <= kEps * dd) break;
             }
             if (mm == l) break;  // d[l] is an eigenvalue
             if (++iter > maxit) {
                 failed = true;
                 break;
             }
-            const double dl = sd[l][lane], el = se[l][lane];
-            double gg = (sd[l + 1][lane] - dl) * rcp_nr(2.0 * el);
+            const double dl = sd_(l), el = se_(l);
+            double gg = (sd_(l + 1) - dl) * rcp_nr(2.0 * el);
             double r = lapy2(gg, 1.0);
-            gg = sd[mm][lane] - dl + el * rcp_nr(gg + copysign(r, gg));
+            gg = sd_(mm) - dl + el * rcp_nr(gg + copysign(r, gg));
             double sn = 1.0, c = 1.0, p = 0.0;
             bool underflow = false;
             // d[k+1] and the two rows' entries k+1 ride in registers from one
             // step to the next; each step loads entry k and stores entry k+1
-            double dk1 = sd[mm][lane], x1 = s0[mm][lane], y1 = s1[mm][lane];
+            double dk1 = sd_(mm), x1 = s0_(mm), y1 = s1_(mm);
             for (int k = mm - 1; k >= l; k--) {
-                const double ek = se[k][lane], dk = sd[k][lane];
-                const double x0 = s0[k][lane], y0 = s1[k][lane];
+                const double ek = se_(k), dk = sd_(k);
+                const double x0 = s0_(k), y0 = s1_(k);
                 const double f = sn * ek, b = c * ek;
                 r = lapy2(f, gg);
-                se[k + 1][lane] = r;
+                se_(k + 1) = r;
                 if (r == 0.0) {  // recover from underflow
-                    sd[k + 1][lane] = dk1 - p;
-                    s0[k + 1][lane] = x1;
-                    s1[k + 1][lane] = y1;
-                    se[mm][lane] = 0.0;
+                    sd_(k + 1) = dk1 - p;
+                    s0_(k + 1) = x1;
+                    s1_(k + 1) = y1;
+                    se_(mm) = 0.0;
                     underflow = true;
                     break;
                 }
@@ -320,41 +333,45 @@ __global__ __launch_bounds__(leaf_lanes<MAXM>()) void k_leaf(
                 gg = dk1 - p;
                 r = (dk - gg) * sn + 2.0 * c * b;
                 p = sn * r;
-                sd[k + 1][lane] = gg + p;
+                sd_(k + 1) = gg + p;
                 gg = c * r - b;
                 // the two kept rows of the eigenvector matrix
-                s0[k + 1][lane] = sn * x0 + c * x1;
+                s0_(k + 1) = sn * x0 + c * x1;
                 x1 = c * x0 - sn * x1;
-                s1[k + 1][lane] = sn * y0 + c * y1;
+                s1_(k + 1) = sn * y0 + c * y1;
                 y1 = c * y0 - sn * y1;
                 dk1 = dk;
             }
             if (underflow) continue;
-            s0[l][lane] = x1;
-            s1[l][lane] = y1;
-            sd[l][lane] = dk1 - p;
-            se[l][lane] = gg;
-            se[mm][lane] = 0.0;
+            s0_(l) = x1;
+            s1_(l) = y1;
+            sd_(l) = dk1 - p;
+            se_(l) = gg;
+            se_(mm) = 0.0;
         }
     }
     if (failed) report_item_failure(fail_site(info, moff, n, it), m);
     // ascending order by rank (ties by position), carrying the two rows
     for (int a = 0; a < m; a++) {
-        const double x = sd[a][lane];
+        const double x = sd_(a);
         int rank = 0;
         for (int b = 0; b < m; b++) {
-            const double y = sd[b][lane];
+            const double y = sd_(b);
             rank += (y < x) || (y == x && b < a);
         }
         if (single) {
             out_final[s + rank] = x * scal[2 * it.mat + 1];
         } else {
             out_d[s + rank] = x;
-            out_lo[s + rank] = s0[a][lane];
-            out_hi[s + rank] = s1[a][lane];
+            out_lo[s + rank] = s0_(a);
+            out_hi[s + rank] = s1_(a);
         }
     }
     if (single && info_user) *info_user = *info;  // a one-leaf solve: one matrix
+#undef sd_
+#undef se_
+#undef s0_
+#undef s1_
 }
 
 // ---------------------------------------------------------------------------

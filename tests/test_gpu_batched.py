@@ -90,6 +90,20 @@ def test_batched_workload_2048(gpu, orc):
         assert rel_err(ws[b], ref) <= TOL, b
 
 
+def test_many_small_matrices_packed_roots(gpu, orc):
+    """6000 matrices of 20..60 rows: the bottom heights hold thousands of small
+    merges, so their secular roots run packed across merges (one wave spans
+    several merges' roots).  Every matrix against the oracle."""
+    rng = np.random.default_rng(12)
+    sizes = [int(x) for x in rng.integers(20, 61, 6000)]
+    mats = [generate(KINDS[b % 4], n, seed_for(12, n, b)) for b, n in enumerate(sizes)]
+    ws, info, bs = solve_batch(gpu, mats)
+    assert np.all(info == 0)
+    for b, ((d, e), w) in enumerate(zip(mats, ws)):
+        ref, _ = orc.dstebr(d, e)
+        assert rel_err(w, ref) <= TOL, b
+
+
 def test_per_matrix_info_and_empty_matrices(gpu, orc):
     mats = [generate("normal", n, seed_for(8, n, b)) for b, n in enumerate([50, 0, 300, 0, 20])]
     bad = mats[2][0].copy()
