@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/brdc.h"
+#include "brdc_deflate_wide.h"
 #include "brdc_grad.h"
 #include "brdc_kernels.h"
 
@@ -74,6 +75,7 @@ struct Layout {
     size_t rlam, rs, rn1, rn2;
     size_t pw, qo1, qo2;
     size_t P, rend, fl;
+    size_t wrec, wtol;  // chunk records and tolerances of the wide deflation
     size_t counts, info;
     size_t plan, moff, tpre, poff;  // batch only
     size_t total;
@@ -120,6 +122,10 @@ Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items) 
     L.P = take(ni);
     L.rend = take(ni);
     L.fl = take(ni);
+    // a wide level has at most kWideMaxMerges merges of at most rows rows
+    L.wrec = take((size_t)kWideFields * kWideMaxMerges *
+                  ((size_t)std::max(rows, 1) / kWideChunk + 2) * sizeof(int));
+    L.wtol = take((size_t)kWideMaxMerges * sizeof(double));
     L.counts = take((size_t)kCountStride * std::max<size_t>(merge_records, 1) * sizeof(int));
     L.info = take(sizeof(int));
     L.plan = plan_items ? take(plan_items * sizeof(int4)) : 0;
@@ -332,7 +338,26 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
                                slo[cur], shi[cur], ew, md, mz, mq1, mq2);
         }
         BRDC_LAUNCH_CHECK();
-        {
+        if (merges <= kWideMaxMerges && maxsz >= kWideMinSize) {
+            // few huge merges: the deflation runs chunked over many blocks
+            ProfScope ps(kDeflate, st);
+            const int chunks = (maxsz + kWideChunk - 1) / kWideChunk;
+            const WideGeo wg{g, chunks, at<int>(ws, L.wrec), at<double>(ws, L.wtol)};
+            const dim3 grid(merges * chunks), blk(kWideThreads);
+            hipLaunchKernelGGL(k_wide_flags, grid, blk, 0, st, wg, (const double*)ew,
+                               (const double*)md, (const double*)mz, fl);
+            hipLaunchKernelGGL(k_wide_compact, grid, blk, 0, st, wg, (const int*)fl, P);
+            hipLaunchKernelGGL(k_wide_spec, grid, blk, 0, st, wg, (const int*)P, (const double*)md,
+                               (const double*)mz, rend);
+            hipLaunchKernelGGL(k_wide_resolve, dim3(merges), blk, 0, st, wg, (const int*)P,
+                               (const double*)md, (const double*)mz, rend, counts);
+            hipLaunchKernelGGL(k_wide_replay, grid, blk, 0, st, wg, (const int*)P,
+                               (const int*)rend, md, mz, mq1, mq2, fl, gd, gq1, gq2);
+            hipLaunchKernelGGL(k_wide_count, grid, blk, 0, st, wg, (const int*)fl);
+            hipLaunchKernelGGL(k_wide_scatter, grid, blk, 0, st, wg, (const int*)fl,
+                               (const double*)md, (const double*)mz, (const double*)mq1,
+                               (const double*)mq2, ad, az, adz, aq, td, tq1, tq2, counts);
+        } else {
             ProfScope ps(kDeflate, st);
             launch_deflate(maxsz, merges, st, g, ew, md, mz, mq1, mq2, P, rend, fl, ad, az, adz,
                            aq, gd, gq1, gq2, td, tq1, tq2, counts);
