@@ -1478,15 +1478,30 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     auto range = [&](int ja, int jb, bool check) {
         int j = ja;
         for (; j + kBatch <= jb; j += kBatch) {
+            // one base per batch, so the batch arrives in a few wide scalar loads
+            const cdbl rb = rst.p + 2 * (size_t)j;
+            const cdbl db = d.p + j;
             double2 r[kBatch];
             double dj[kBatch];
 #pragma unroll
             for (int b = 0; b < kBatch; b++) {
-                r[b] = rst[j + b];
-                dj[b] = d[j + b];
+                r[b] = make_double2(rb[2 * b], rb[2 * b + 1]);
+                dj[b] = db[b];
             }
+            // even and odd pairs in two product chains, joined before the
+            // exponents move out
+            const double n0 = num, d0 = den;
+            num = 1.0;
+            den = 1.0;
 #pragma unroll
-            for (int b = 0; b < kBatch; b++) term(r[b], dj[b], j + b, check);
+            for (int b = 0; b < kBatch; b += 2) term(r[b], dj[b], j + b, check);
+            const double n1 = num, d1 = den;
+            num = 1.0;
+            den = 1.0;
+#pragma unroll
+            for (int b = 1; b < kBatch; b += 2) term(r[b], dj[b], j + b, check);
+            num = (n0 * n1) * num;
+            den = (d0 * d1) * den;
             renorm();
         }
         for (; j < jb; j++) {
@@ -1583,11 +1598,14 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
     auto range = [&](int ia, int ib, bool check) {
         int i = ia;
         for (; i + kBatch <= ib; i += kBatch) {
+            // one base per batch, so the batch arrives in a few wide scalar loads
+            const cdbl pb = P.p + 2 * (size_t)i;
+            const cdbl qb = Q.p + 2 * (size_t)i;
             double2 p[kBatch], q[kBatch];
 #pragma unroll
             for (int b = 0; b < kBatch; b++) {
-                p[b] = P[i + b];
-                q[b] = Q[i + b];
+                p[b] = make_double2(pb[2 * b], pb[2 * b + 1]);
+                q[b] = make_double2(qb[2 * b], qb[2 * b + 1]);
             }
 #pragma unroll
             for (int b = 0; b < kBatch; b += 2) {
