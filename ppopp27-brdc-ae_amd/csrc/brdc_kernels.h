@@ -1217,20 +1217,26 @@ __device__ __forceinline__ void stream_side(const double2* __restrict__ dz, doub
             // J gets sum_b (j + b L) t_b = j * T + L * sum_b b t_b
             double T = 0.0, B = 0.0;
 #pragma unroll
-            for (int b = 0; b < kStreamBatch; b += 2) {
-                // one reciprocal for two poles: 1/x = y/(xy), 1/y = x/(xy)
-                // (deltas of the scaled problem lie far inside the range
-                // where xy neither overflows nor underflows)
-                const double x = (p[b].x - org) - tau, y = (p[b + 1].x - org) - tau;
-                const double rxy = rcp_stream(x * y);
-                const double r0 = y * rxy, r1 = x * rxy;
-                const double t0 = p[b].y * r0, t1 = p[b + 1].y * r1;
-                T += t0;
-                T += t1;
-                a.D = fma(t0, r0, a.D);
-                a.D = fma(t1, r1, a.D);
-                if (b > 0) B = fma((double)b, t0, B);
-                B = fma((double)(b + 1), t1, B);
+            for (int b = 0; b < kStreamBatch; b += 4) {
+                // one reciprocal for four poles: with u = xy, v = zw,
+                // 1/u = v/(uv) and 1/v = u/(uv), then 1/x = y/u and so on.
+                // Poles closer than the deflation tolerance were deflated, so
+                // every delta here is at least about 1e-16 and at most the
+                // spectrum's width: uv neither overflows nor underflows.
+                double dl[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) dl[q] = (p[b + q].x - org) - tau;
+                const double u = dl[0] * dl[1], v = dl[2] * dl[3];
+                const double ruv = rcp_stream(u * v);
+                const double ru = v * ruv, rv = u * ruv;
+                const double r[4] = {dl[1] * ru, dl[0] * ru, dl[3] * rv, dl[2] * rv};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double t = p[b + q].y * r[q];
+                    T += t;
+                    a.D = fma(t, r[q], a.D);
+                    if (b + q > 0) B = fma((double)(b + q), t, B);
+                }
             }
             a.S += T;
             a.J = fma((double)(c0 + o), T, a.J);
@@ -1608,12 +1614,19 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
                 q[b] = make_double2(qb[2 * b], qb[2 * b + 1]);
             }
 #pragma unroll
-            for (int b = 0; b < kBatch; b += 2) {
-                // one reciprocal for two poles: 1/u = v/(uv), 1/v = u/(uv)
-                const double u = delta(p[b], i + b, check), v = delta(p[b + 1], i + b + 1, check);
-                const double ruv = rcp_stream(u * v);
-                accumulate(p[b].y * (v * ruv), q[b]);
-                accumulate(p[b + 1].y * (u * ruv), q[b + 1]);
+            for (int b = 0; b < kBatch; b += 4) {
+                // one reciprocal for four poles, as in the streamed secular
+                // kernel (deltas of undeflated poles keep the product in range)
+                double dl[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) dl[u] = delta(p[b + u], i + b + u, check);
+                const double x01 = dl[0] * dl[1], x23 = dl[2] * dl[3];
+                const double rq = rcp_stream(x01 * x23);
+                const double r01 = x23 * rq, r23 = x01 * rq;
+                accumulate(p[b].y * (dl[1] * r01), q[b]);
+                accumulate(p[b + 1].y * (dl[0] * r01), q[b + 1]);
+                accumulate(p[b + 2].y * (dl[3] * r23), q[b + 2]);
+                accumulate(p[b + 3].y * (dl[2] * r23), q[b + 3]);
             }
         }
         for (; i < ib; i++) accumulate(P[i].y * rcp_stream(delta(P[i], i, check)), Q[i]);
