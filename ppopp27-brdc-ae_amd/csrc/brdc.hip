@@ -77,11 +77,12 @@ struct Layout {
     size_t P, rend, fl;
     size_t wrec, wtol;  // chunk records and tolerances of the wide deflation
     size_t counts, info;
-    size_t plan, moff, tpre, poff;  // batch only
+    size_t plan, moff, tpre, poff, tord;  // batch only
     size_t total;
 };
 
-Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items) {
+Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items,
+                   size_t grad_tasks = 0) {
     Layout L{};
     const size_t nd = (size_t)std::max(rows, 1) * sizeof(double);
     const size_t ni = (size_t)std::max(rows, 1) * sizeof(int);
@@ -132,6 +133,7 @@ Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items) 
     L.moff = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
     L.tpre = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
     L.poff = plan_items ? take(((size_t)nmat + 1) * sizeof(long long)) : 0;
+    L.tord = plan_items ? take(std::max<size_t>(grad_tasks, 1) * sizeof(int)) : 0;
     L.total = off;
     return L;
 }
@@ -458,6 +460,7 @@ struct brdc_batch_plan {
     std::vector<int> moff;           // nmat + 1
     std::vector<int> tprefix;        // gradient tasks (64 eigenvalues) before each matrix
     std::vector<long long> poff;     // gradient partial-sum offset of each matrix
+    std::vector<int> torder;         // gradient tasks, longest matrix first
     int rows_max = 0;
     std::vector<int4> items;         // leaves, then every level's merges
     int nleaves = 0, leaf_rows = 0;
@@ -497,7 +500,8 @@ GradShape grad_shape(int ntask, int rows_max, long long partial_doubles) {
 }
 
 int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const long long* poff,
-                 int ntask, int rows_max, long long partial_doubles, const double* d,
+                 const int* torder, int ntask, int rows_max, long long partial_doubles,
+                 const double* d,
                  const double* e, const double* w, const double* g, double* dd, double* de,
                  void* scratch, hipStream_t st) {
     const GradShape G = grad_shape(ntask, rows_max, partial_doubles);
@@ -506,8 +510,8 @@ int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const 
     const int n_one = moff ? 0 : rows;
     if (ntask > 0) {
         hipLaunchKernelGGL(k_grad_vectors, dim3(G.waves / kGradWaves), dim3(kWave * kGradWaves),
-                           0, st, nmat, moff, tprefix, poff, ntask, n_one, d, e, w, g, cp, G.slot,
-                           part);
+                           0, st, nmat, moff, tprefix, poff, torder, ntask, n_one, d, e, w, g, cp,
+                           G.slot, part);
         BRDC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
                            moff, tprefix, poff, n_one, (const double*)part, dd, de);
@@ -776,7 +780,19 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
         p->merge_records += (size_t)L.merges;
         p->levels.push_back(L);
     }
-    p->layout = make_layout(p->rows, nmat, p->merge_records, std::max<size_t>(p->items.size(), 1));
+    // gradient tasks longest first: matrices by descending order, ties by index
+    {
+        std::vector<int> mats((size_t)nmat);
+        for (int b = 0; b < nmat; b++) mats[(size_t)b] = b;
+        std::stable_sort(mats.begin(), mats.end(),
+                         [&](int a, int b) { return sizes[a] > sizes[b]; });
+        p->torder.clear();
+        p->torder.reserve((size_t)p->tprefix.back());
+        for (int b : mats)
+            for (int t = p->tprefix[b]; t < p->tprefix[b + 1]; t++) p->torder.push_back(t);
+    }
+    p->layout = make_layout(p->rows, nmat, p->merge_records, std::max<size_t>(p->items.size(), 1),
+                            p->torder.size());
     *plan = p;
     return BRDC_STATUS_SUCCESS;
 }
@@ -816,6 +832,10 @@ int brdc_batch_upload(const brdc_batch_plan* plan, void* workspace, size_t works
         hipMemcpyAsync(at<long long>(workspace, L.poff), plan->poff.data(),
                        plan->poff.size() * sizeof(long long), hipMemcpyHostToDevice, st) !=
             hipSuccess ||
+        (!plan->torder.empty() &&
+         hipMemcpyAsync(at<int>(workspace, L.tord), plan->torder.data(),
+                        plan->torder.size() * sizeof(int), hipMemcpyHostToDevice, st) !=
+             hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
         return BRDC_STATUS_HIP_ERROR;
     return BRDC_STATUS_SUCCESS;
@@ -853,8 +873,8 @@ int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
     if (scratch_bytes < grad_shape(tasks, n, partial).total ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
-    return enqueue_grad(n, 1, nullptr, nullptr, nullptr, tasks, n, partial, d_dev, e_dev, w_dev,
-                        g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
+    return enqueue_grad(n, 1, nullptr, nullptr, nullptr, nullptr, tasks, n, partial, d_dev, e_dev,
+                        w_dev, g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
 }
 
 int brdc_batch_grad_workspace(const brdc_batch_plan* plan, size_t* bytes) {
@@ -880,7 +900,8 @@ int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const doub
         return BRDC_STATUS_INVALID_WORKSPACE;
     void* ws = const_cast<void*>(workspace);
     return enqueue_grad(plan->rows, plan->nmat, at<int>(ws, L.moff), at<int>(ws, L.tpre),
-                        at<long long>(ws, L.poff), ntask, plan->rows_max, partial, d_dev, e_dev,
+                        at<long long>(ws, L.poff), at<int>(ws, L.tord), ntask, plan->rows_max,
+                        partial, d_dev, e_dev,
                         w_dev, g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
 }
 

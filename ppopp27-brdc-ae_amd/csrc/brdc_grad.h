@@ -89,11 +89,17 @@ __device__ __forceinline__ double reduce_scatter(double* v, int lane) {
 //   cp: per wave 3 arrays of (segments + 1) x 64 doubles: cpF[s] = D+ and
 //       cpB[s] = D- at row s*kSeg, zc[s] = z at row s*kSeg.
 //   partial: per task 2 * n_b doubles at poff[b] + (t - tprefix[b]) * 2 * n_b.
+//   torder: the tasks longest first (a batch's tasks by descending matrix
+//       order), or nullptr for task order.  Wave w takes positions w, then
+//       2 nwave - 1 - w, then 2 nwave + w, ... of that list, so the longest
+//       tasks go one per wave and the second round starts on the waves that
+//       hold the shortest first tasks.
 __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     int nmat, const int* __restrict__ moff, const int* __restrict__ tprefix,
-    const long long* __restrict__ poff, int ntask, int n_one, const double* __restrict__ d,
-    const double* __restrict__ e, const double* __restrict__ w, const double* __restrict__ g,
-    double* __restrict__ cpbuf, long long slot, double* __restrict__ partial) {
+    const long long* __restrict__ poff, const int* __restrict__ torder, int ntask, int n_one,
+    const double* __restrict__ d, const double* __restrict__ e, const double* __restrict__ w,
+    const double* __restrict__ g, double* __restrict__ cpbuf, long long slot,
+    double* __restrict__ partial) {
     __shared__ double stage[kGradWaves][2][kWin];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = uniform((int)(threadIdx.x / kWave));
@@ -102,7 +108,10 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     const double safmin = 0x1p-1022;
     double* sD = stage[wv][0];
     double* sE = stage[wv][1];
-    for (int t = wave; t < ntask; t += nwave) {
+    for (int round = 0;; round++) {
+        const int q = round * nwave + ((round & 1) ? nwave - 1 - wave : wave);
+        if (q >= ntask) break;  // positions grow with the round
+        const int t = torder ? uniform(torder[q]) : q;
         const int b = uniform(grad_task_matrix(tprefix, nmat, t));
         const int off = moff ? uniform(moff[b]) : 0;
         const int n = moff ? uniform(moff[b + 1]) - off : n_one;
