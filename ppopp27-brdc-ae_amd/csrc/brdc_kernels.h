@@ -1001,33 +1001,44 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
 // not a multiple of the group.  k_root_scan forms roff (exclusive prefix of
 // the merges' K, roff[merges] = total) in one block.
 constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 4;  // consecutive merges per thread in a tile
 __global__ __launch_bounds__(kScanThreads) void k_root_scan(int merges, int kstream,
                                                              const int* __restrict__ counts,
                                                              int* __restrict__ roff) {
-    __shared__ int part[kScanThreads];
-    const int t = threadIdx.x;
-    const int per = (merges + kScanThreads - 1) / kScanThreads;
-    const int m0 = min(merges, t * per), m1 = min(merges, m0 + per);
-    auto roots = [&](int m) {
-        const int K = counts[kCountStride * m];
-        return K < kstream ? K : 0;
-    };
-    int sum = 0;
-    for (int m = m0; m < m1; m++) sum += roots(m);
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < kScanThreads; o <<= 1) {  // inclusive Hillis-Steele scan
-        const int v = (t >= o) ? part[t - o] : 0;
+    constexpr int NW = kScanThreads / kWave;
+    constexpr int kTile = kScanThreads * kScanPer;
+    __shared__ int wsum[NW];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    int carry = 0;  // roots of the earlier tiles
+    for (int base = 0; base < merges; base += kTile) {
+        // this thread's merges: base + t * kScanPer ... (+ kScanPer - 1)
+        int k[kScanPer], sum = 0;
+#pragma unroll
+        for (int u = 0; u < kScanPer; u++) {
+            const int m = base + t * kScanPer + u;
+            const int K = (m < merges) ? counts[kCountStride * m] : 0;
+            k[u] = (K < kstream) ? K : 0;
+            sum += k[u];
+        }
+        const int incl = wave_incl_sum(sum);
+        if (lane == kWave - 1) wsum[w] = incl;
         __syncthreads();
-        part[t] += v;
-        __syncthreads();
+        int off = carry + incl - sum, tile = 0;
+        for (int q = 0; q < NW; q++) {
+            const int x = wsum[q];
+            if (q < w) off += x;
+            tile += x;
+        }
+#pragma unroll
+        for (int u = 0; u < kScanPer; u++) {
+            const int m = base + t * kScanPer + u;
+            if (m < merges) roff[m] = off;
+            off += k[u];
+        }
+        carry += tile;
+        __syncthreads();  // wsum is rewritten by the next tile
     }
-    int off = part[t] - sum;
-    for (int m = m0; m < m1; m++) {
-        roff[m] = off;
-        off += roots(m);
-    }
-    if (t == kScanThreads - 1) roff[merges] = part[t];
+    if (t == 0) roff[merges] = carry;
 }
 
 // All roots of the level's merges with K < kstream, packed (see k_root_scan):
