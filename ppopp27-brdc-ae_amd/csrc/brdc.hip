@@ -77,12 +77,12 @@ struct Layout {
     size_t P, rend, fl;
     size_t wrec, wtol;  // chunk records and tolerances of the wide deflation
     size_t counts, info;
-    size_t plan, moff, tpre, poff, tord;  // batch only
+    size_t plan, moff, tpre, poff, cpof, gitm;  // batch only
     size_t total;
 };
 
 Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items,
-                   size_t grad_tasks = 0) {
+                   size_t grad_items = 0) {
     Layout L{};
     const size_t nd = (size_t)std::max(rows, 1) * sizeof(double);
     const size_t ni = (size_t)std::max(rows, 1) * sizeof(int);
@@ -133,7 +133,8 @@ Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items,
     L.moff = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
     L.tpre = plan_items ? take(((size_t)nmat + 1) * sizeof(int)) : 0;
     L.poff = plan_items ? take(((size_t)nmat + 1) * sizeof(long long)) : 0;
-    L.tord = plan_items ? take(std::max<size_t>(grad_tasks, 1) * sizeof(int)) : 0;
+    L.cpof = plan_items ? take(((size_t)nmat + 1) * sizeof(long long)) : 0;
+    L.gitm = plan_items ? take(std::max<size_t>(grad_items, 1) * sizeof(int4)) : 0;
     L.total = off;
     return L;
 }
@@ -460,7 +461,9 @@ struct brdc_batch_plan {
     std::vector<int> moff;           // nmat + 1
     std::vector<int> tprefix;        // gradient tasks (64 eigenvalues) before each matrix
     std::vector<long long> poff;     // gradient partial-sum offset of each matrix
-    std::vector<int> torder;         // gradient tasks, longest matrix first
+    std::vector<long long> cpoff;    // gradient checkpoint offset of each matrix
+    std::vector<int4> gitems;        // gradient items of the four launches, back to back
+    int gcount[4] = {0, 0, 0, 0};    // items per launch
     int rows_max = 0;
     std::vector<int4> items;         // leaves, then every level's merges
     int nleaves = 0, leaf_rows = 0;
@@ -472,52 +475,87 @@ struct brdc_batch_plan {
 namespace brdc {
 namespace {
 
-// Gradient scratch: per wave three checkpoint arrays of (segments + 1) x 64
-// doubles, then the per-task partial sums.
+// Gradient scratch: per task three checkpoint arrays of (segments + 1) x 64
+// doubles; per task 64 twist indices and two |z|^2 halves; per gamma item 64
+// best |gamma| values and rows; then the per-task partial sums.
 struct GradShape {
-    int waves;
-    long long slot;  // doubles of checkpoints per wave
-    size_t cp, partial, total;
+    size_t cp, rarr, ndarr, nuarr, garr, rgarr, partial, total;
 };
 
-GradShape grad_shape(int ntask, int rows_max, long long partial_doubles) {
+// checkpoint doubles of one task of an n-row matrix
+inline long long grad_slot(int n) {
+    const long long segs = (std::max(n, 1) + kSeg - 1) / kSeg;
+    return 3 * (segs + 1) * kWave;
+}
+
+GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles, int ngamma) {
     GradShape G{};
-    const long long segs = (std::max(rows_max, 1) + kSeg - 1) / kSeg;
-    G.slot = 3 * (segs + 1) * kWave;
-    // one wave per task up to a few resident waves per SIMD
-    G.waves = std::max(kGradWaves, std::min((ntask + kGradWaves - 1) / kGradWaves * kGradWaves,
-                                            4096));
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t at = off;
         off = align_up(off + bytes);
         return at;
     };
-    G.cp = take((size_t)G.waves * G.slot * sizeof(double));
+    const size_t lanes = (size_t)std::max(ntask, 1) * kWave;
+    const size_t glanes = (size_t)std::max(ngamma, 1) * kWave;
+    G.cp = take((size_t)std::max<long long>(cp_doubles, 1) * sizeof(double));
+    G.rarr = take(lanes * sizeof(int));
+    G.ndarr = take(lanes * sizeof(double));
+    G.nuarr = take(lanes * sizeof(double));
+    G.garr = take(glanes * sizeof(double));
+    G.rgarr = take(glanes * sizeof(int));
     G.partial = take((size_t)std::max<long long>(partial_doubles, 1) * sizeof(double));
     G.total = off;
     return G;
 }
 
+// The four launches of the eigenvector kernel (chains and short matrices;
+// twist search; z; contributions), then the per-matrix sums.  items: the
+// four item lists back to back (count[l] each), or nullptr for one matrix,
+// whose items follow from the task count and parts.
 int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const long long* poff,
-                 const int* torder, int ntask, int rows_max, long long partial_doubles,
-                 const double* d,
+                 const long long* cpoff, const int4* items, const int count[4], int parts,
+                 int ntask, long long cp_doubles, long long partial_doubles, const double* d,
                  const double* e, const double* w, const double* g, double* dd, double* de,
                  void* scratch, hipStream_t st) {
-    const GradShape G = grad_shape(ntask, rows_max, partial_doubles);
+    const GradShape G = grad_shape(ntask, cp_doubles, partial_doubles, count[1]);
     double* cp = at<double>(scratch, G.cp);
+    int* rarr = at<int>(scratch, G.rarr);
+    double* ndarr = at<double>(scratch, G.ndarr);
+    double* nuarr = at<double>(scratch, G.nuarr);
+    double* garr = at<double>(scratch, G.garr);
+    int* rgarr = at<int>(scratch, G.rgarr);
     double* part = at<double>(scratch, G.partial);
     const int n_one = moff ? 0 : rows;
-    if (ntask > 0) {
-        hipLaunchKernelGGL(k_grad_vectors, dim3(G.waves / kGradWaves), dim3(kWave * kGradWaves),
-                           0, st, nmat, moff, tprefix, poff, torder, ntask, n_one, d, e, w, g, cp,
-                           G.slot, part);
-        BRDC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
-                           moff, tprefix, poff, n_one, (const double*)part, dd, de);
-        BRDC_LAUNCH_CHECK();
+    if (ntask <= 0) return BRDC_STATUS_SUCCESS;
+    size_t base = 0;
+    for (int l = 0; l < 4; l++) {
+        const int cnt = count[l];
+        if (cnt > 0) {  // a wave per item, up to a few per SIMD
+            const int waves = std::min(cnt, 4096);
+            hipLaunchKernelGGL(k_grad_vectors, dim3((waves + kGradWaves - 1) / kGradWaves),
+                               dim3(kWave * kGradWaves), 0, st, l + 1,
+                               items ? items + base : (const int4*)nullptr, cnt, parts, nmat, moff,
+                               tprefix, poff, cpoff, n_one, d, e, w, g, cp, rarr, ndarr, nuarr,
+                               garr, rgarr, part);
+            BRDC_LAUNCH_CHECK();
+        }
+        base += (size_t)std::max(cnt, 0);
     }
+    hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
+                       moff, tprefix, poff, n_one, (const double*)part, dd, de);
+    BRDC_LAUNCH_CHECK();
     return BRDC_STATUS_SUCCESS;
+}
+
+// item counts of one matrix of order n (tasks of 64 eigenvalues)
+inline void grad_counts_one(int n, int count[4]) {
+    const int tasks = (n + kWave - 1) / kWave, parts = grad_parts(n);
+    const bool split = parts > 1;
+    count[0] = split ? 2 * tasks : tasks;
+    count[1] = split ? tasks * parts : 0;
+    count[2] = split ? 2 * tasks : 0;
+    count[3] = split ? tasks * parts : 0;
 }
 
 Schedule batch_schedule(const brdc_batch_plan& p, void* ws) {
@@ -780,19 +818,51 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
         p->merge_records += (size_t)L.merges;
         p->levels.push_back(L);
     }
-    // gradient tasks longest first: matrices by descending order, ties by index
+    // gradient items: matrices by descending order (ties by index) in every
+    // launch, so the longest work comes first; a long matrix's tasks split
+    // into chains, twist-search ranges, z directions and contribution ranges
     {
         std::vector<int> mats((size_t)nmat);
         for (int b = 0; b < nmat; b++) mats[(size_t)b] = b;
         std::stable_sort(mats.begin(), mats.end(),
                          [&](int a, int b) { return sizes[a] > sizes[b]; });
-        p->torder.clear();
-        p->torder.reserve((size_t)p->tprefix.back());
-        for (int b : mats)
-            for (int t = p->tprefix[b]; t < p->tprefix[b + 1]; t++) p->torder.push_back(t);
+        std::vector<int4> L1, L2, L3, L4;
+        std::vector<int4>* lists[4] = {&L1, &L2, &L3, &L4};
+        int shorts = 0;
+        for (int b : mats) {
+            const int segs = (sizes[b] + kSeg - 1) / kSeg, parts = grad_parts(sizes[b]);
+            for (int t = p->tprefix[b]; t < p->tprefix[b + 1]; t++) {
+                if (parts == 1) {
+                    // a short matrix's task runs whole, dealt round-robin over
+                    // the four launches beside the long matrices' items
+                    lists[shorts++ & 3]->push_back(make_int4(t, kGAll, 0, 0));
+                    continue;
+                }
+                L1.push_back(make_int4(t, kGChainF, 0, 0));
+                L1.push_back(make_int4(t, kGChainB, 0, 0));
+                const int g0 = (int)L2.size();
+                for (int c = 0; c < parts; c++) {
+                    const int sa = (int)((long long)segs * c / parts);
+                    const int sb = (int)((long long)segs * (c + 1) / parts);
+                    L2.push_back(make_int4(t, kGGamma, sa, sb));
+                    L4.push_back(make_int4(t, kGContrib, sa, sb));
+                }
+                L3.push_back(make_int4(t, kGZDown, g0, parts));
+                L3.push_back(make_int4(t, kGZUp, g0, parts));
+            }
+        }
+        p->gitems.clear();
+        for (int l = 0; l < 4; l++) {
+            p->gcount[l] = (int)lists[l]->size();
+            p->gitems.insert(p->gitems.end(), lists[l]->begin(), lists[l]->end());
+        }
+        p->cpoff.assign((size_t)nmat + 1, 0);
+        for (int b = 0; b < nmat; b++)
+            p->cpoff[b + 1] = p->cpoff[b] +
+                              (long long)(p->tprefix[b + 1] - p->tprefix[b]) * grad_slot(sizes[b]);
     }
     p->layout = make_layout(p->rows, nmat, p->merge_records, std::max<size_t>(p->items.size(), 1),
-                            p->torder.size());
+                            p->gitems.size());
     *plan = p;
     return BRDC_STATUS_SUCCESS;
 }
@@ -832,10 +902,13 @@ int brdc_batch_upload(const brdc_batch_plan* plan, void* workspace, size_t works
         hipMemcpyAsync(at<long long>(workspace, L.poff), plan->poff.data(),
                        plan->poff.size() * sizeof(long long), hipMemcpyHostToDevice, st) !=
             hipSuccess ||
-        (!plan->torder.empty() &&
-         hipMemcpyAsync(at<int>(workspace, L.tord), plan->torder.data(),
-                        plan->torder.size() * sizeof(int), hipMemcpyHostToDevice, st) !=
+        (!plan->gitems.empty() &&
+         hipMemcpyAsync(at<int4>(workspace, L.gitm), plan->gitems.data(),
+                        plan->gitems.size() * sizeof(int4), hipMemcpyHostToDevice, st) !=
              hipSuccess) ||
+        hipMemcpyAsync(at<long long>(workspace, L.cpof), plan->cpoff.data(),
+                       plan->cpoff.size() * sizeof(long long), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return BRDC_STATUS_HIP_ERROR;
     return BRDC_STATUS_SUCCESS;
@@ -857,7 +930,9 @@ int brdc_dstebr_grad_workspace(int n, size_t* bytes) {
     if (!bytes) return BRDC_STATUS_INVALID_POINTER;
     if (n < 0) return BRDC_STATUS_INVALID_VALUE;
     const int tasks = (n + kWave - 1) / kWave;
-    *bytes = grad_shape(tasks, n, (long long)tasks * 2 * n).total;
+    int count[4];
+    grad_counts_one(n, count);
+    *bytes = grad_shape(tasks, tasks * grad_slot(n), (long long)tasks * 2 * n, count[1]).total;
     return BRDC_STATUS_SUCCESS;
 }
 
@@ -869,17 +944,21 @@ int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
     if (!d_dev || !e_dev || !w_dev || !g_dev || !dd_dev || !de_dev || !scratch)
         return BRDC_STATUS_INVALID_POINTER;
     const int tasks = (n + kWave - 1) / kWave;
-    const long long partial = (long long)tasks * 2 * n;
-    if (scratch_bytes < grad_shape(tasks, n, partial).total ||
+    const long long partial = (long long)tasks * 2 * n, cpd = tasks * grad_slot(n);
+    int count[4];
+    grad_counts_one(n, count);
+    if (scratch_bytes < grad_shape(tasks, cpd, partial, count[1]).total ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
-    return enqueue_grad(n, 1, nullptr, nullptr, nullptr, nullptr, tasks, n, partial, d_dev, e_dev,
-                        w_dev, g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
+    return enqueue_grad(n, 1, nullptr, nullptr, nullptr, nullptr, nullptr, count, grad_parts(n),
+                        tasks, cpd, partial, d_dev, e_dev, w_dev, g_dev, dd_dev, de_dev,
+                        scratch, static_cast<hipStream_t>(stream));
 }
 
 int brdc_batch_grad_workspace(const brdc_batch_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return BRDC_STATUS_INVALID_POINTER;
-    *bytes = grad_shape(plan->tprefix.back(), plan->rows_max, plan->poff.back()).total;
+    *bytes = grad_shape(plan->tprefix.back(), plan->cpoff.back(), plan->poff.back(),
+                        plan->gcount[1]).total;
     return BRDC_STATUS_SUCCESS;
 }
 
@@ -894,14 +973,15 @@ int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const doub
     const Layout& L = plan->layout;
     const int ntask = plan->tprefix.back();
     const long long partial = plan->poff.back();
-    if (workspace_bytes < L.total || scratch_bytes < grad_shape(ntask, plan->rows_max,
-                                                                  partial).total ||
+    if (workspace_bytes < L.total ||
+        scratch_bytes < grad_shape(ntask, plan->cpoff.back(), partial, plan->gcount[1]).total ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
     void* ws = const_cast<void*>(workspace);
     return enqueue_grad(plan->rows, plan->nmat, at<int>(ws, L.moff), at<int>(ws, L.tpre),
-                        at<long long>(ws, L.poff), at<int>(ws, L.tord), ntask, plan->rows_max,
-                        partial, d_dev, e_dev,
+                        at<long long>(ws, L.poff), at<long long>(ws, L.cpof),
+                        at<int4>(ws, L.gitm), plan->gcount, 1, ntask, plan->cpoff.back(), partial,
+                        d_dev, e_dev,
                         w_dev, g_dev, dd_dev, de_dev, scratch, static_cast<hipStream_t>(stream));
 }
 

@@ -14,7 +14,7 @@
 //
 // Layout: one lane per eigenvalue, 64 eigenvalues of one matrix per wave (a
 // task).  Every lane walks the same row k at the same time, so D and E come
-// in as scalar loads, and the per-lane pivots live in the wave's slot of two
+// in as scalar loads, and the per-lane pivots live in the task's slot of two
 // scratch arrays [row][lane] (coalesced).  Each task writes its partial sums
 // for its matrix's rows; a second kernel adds the tasks of each matrix in a
 // fixed order (deterministic results).
@@ -27,6 +27,14 @@ namespace brdc {
 constexpr int kGradWaves = 4;   // waves per block of the vector kernel
 constexpr int kSeg = 16;        // rows per checkpoint segment
 constexpr int kWin = 256;       // rows of D and E staged in LDS per wave
+// contribution ranges per task of an n-row matrix: about kGradRangeSegs
+// segments each (one range: the chains' launch does the contributions too)
+constexpr int kGradRangeSegs = 256;
+__host__ __device__ inline int grad_parts(int n) {
+    const int segs = (n > 0 ? n : 1) + kSeg - 1;
+    const int p = (segs / kSeg) / kGradRangeSegs;
+    return p < 1 ? 1 : (p > 64 ? 64 : p);
+}
 
 // Task t of a batch: matrix b with tprefix[b] <= t < tprefix[b+1], first
 // eigenvalue 64 * (t - tprefix[b]).  One matrix: tprefix == nullptr.
@@ -77,29 +85,51 @@ __device__ __forceinline__ double reduce_scatter(double* v, int lane) {
     return t;
 }
 
-// One task: 64 eigenvalues of one matrix, one per lane.  The pivots are not
-// stored: pass F keeps the stationary pivot D+ at every segment start (every
-// kSeg rows), pass B the progressive pivot D- likewise, and every later pass
-// recomputes a segment's pivots in registers from its checkpoints.  Per lane
-// the scratch holds three checkpoint rows per segment (D+, D-, z), so memory
+// Work items of the eigenvector kernel (k_grad_vectors).  A task is 64
+// eigenvalues of one matrix, one per lane; its passes are
+//   F  the stationary pivot chain D+ (top down), checkpointed at every
+//      kSeg-row segment start (cpF),
+//   B  the progressive chain D- (bottom up), checkpointed likewise (cpB),
+//   G  gamma_k = D+_k + D-_k - (D_k - mu) and the twist r = argmin |gamma_k|
+//      (segments recomputed from the checkpoints),
+//   Z  z with z_r = 1: downward through D+ below r, upward through D- above
+//      r, with |z|^2 and z at every segment start (zc),
+//   C  the contributions g z_k^2 / |z|^2 and 2 g z_k z_{k+1} / |z|^2, summed
+//      over the wave's eigenvalues per row.
+// A short matrix runs all passes in one item (kGAll).  A long one (grad_parts
+// above 1) spreads them over four launches, so its independent work runs on
+// separate waves: the two chains (kGChainF, kGChainB), the twist search over
+// row ranges (kGGamma), the two z directions (kGZDown, kGZUp), and the
+// contributions over row ranges (kGContrib).
+enum GradKind { kGAll, kGChainF, kGChainB, kGGamma, kGZDown, kGZUp, kGContrib };
+
+// One task's passes over its pivots.  The pivots are not stored: every pass
+// recomputes a segment's pivots in registers from its checkpoints, so memory
 // traffic is O(n / kSeg) per eigenvalue and the passes are bound by their
-// dependent recurrences, not by memory.  All lanes walk the same segment at
-// the same time (per-lane differences are selects), so D and E arrive as
-// scalar loads and the last pass can sum each row over the wave's lanes.
-//   cp: per wave 3 arrays of (segments + 1) x 64 doubles: cpF[s] = D+ and
+// dependent recurrences.  All lanes walk the same segment at the same time
+// (per-lane differences are selects), so D and E come from an LDS window
+// shared by the wave, and the last pass can sum each row over the lanes.
+//   items: the launch's items (task, kind, a, b): a, b = segment range
+//       [sa, sb) (kGGamma, kGContrib) or the task's first gamma item and its
+//       count (kGZDown, kGZUp); nullptr (one matrix of n_one rows): item q of
+//       launch lmode is derived from the task count and parts = grad_parts.
+//   cp: per task 3 arrays of (segments + 1) x 64 doubles at cpoff[b] + (t -
+//       tprefix[b]) * slot_b (cpoff == nullptr: t * slot_b): cpF[s] = D+ and
 //       cpB[s] = D- at row s*kSeg, zc[s] = z at row s*kSeg.
+//   rarr, ndarr, nuarr: per task 64 twists and the two halves of |z|^2;
+//   garr, rgarr: per gamma item 64 best |gamma| values and their rows.
 //   partial: per task 2 * n_b doubles at poff[b] + (t - tprefix[b]) * 2 * n_b.
-//   torder: the tasks longest first (a batch's tasks by descending matrix
-//       order), or nullptr for task order.  Wave w takes positions w, then
-//       2 nwave - 1 - w, then 2 nwave + w, ... of that list, so the longest
-//       tasks go one per wave and the second round starts on the waves that
-//       hold the shortest first tasks.
+// Wave w takes items w, then 2 nwave - 1 - w, then 2 nwave + w, ... so with
+// the longest items listed first they go one per wave, and a second round
+// starts on the waves that hold the shortest first items.
 __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
-    int nmat, const int* __restrict__ moff, const int* __restrict__ tprefix,
-    const long long* __restrict__ poff, const int* __restrict__ torder, int ntask, int n_one,
+    int lmode, const int4* __restrict__ items, int nitem, int parts, int nmat,
+    const int* __restrict__ moff, const int* __restrict__ tprefix,
+    const long long* __restrict__ poff, const long long* __restrict__ cpoff, int n_one,
     const double* __restrict__ d, const double* __restrict__ e, const double* __restrict__ w,
-    const double* __restrict__ g, double* __restrict__ cpbuf, long long slot,
-    double* __restrict__ partial) {
+    const double* __restrict__ g, double* __restrict__ cpbuf, int* __restrict__ rarr,
+    double* __restrict__ ndarr, double* __restrict__ nuarr, double* __restrict__ garr,
+    int* __restrict__ rgarr, double* __restrict__ partial) {
     __shared__ double stage[kGradWaves][2][kWin];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = uniform((int)(threadIdx.x / kWave));
@@ -110,8 +140,31 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     double* sE = stage[wv][1];
     for (int round = 0;; round++) {
         const int q = round * nwave + ((round & 1) ? nwave - 1 - wave : wave);
-        if (q >= ntask) break;  // positions grow with the round
-        const int t = torder ? uniform(torder[q]) : q;
+        if (q >= nitem) break;  // positions grow with the round
+        int t, kind, ia = 0, ib = 0;
+        int S1 = (n_one + kSeg - 1) / kSeg;  // one matrix: its segments
+        if (items) {
+            const int4 it = items[q];
+            t = uniform(it.x);
+            kind = uniform(it.y);
+            ia = uniform(it.z);
+            ib = uniform(it.w);
+        } else if (lmode == 1) {
+            const bool split = parts > 1;
+            t = split ? q >> 1 : q;
+            kind = split ? kGChainF + (q & 1) : kGAll;
+        } else if (lmode == 3) {
+            t = q >> 1;
+            kind = kGZDown + (q & 1);
+            ia = t * parts;
+            ib = parts;
+        } else {
+            t = q / parts;
+            const int c = q - t * parts;
+            kind = (lmode == 2) ? kGGamma : kGContrib;
+            ia = (int)(((long long)S1 * c) / parts);
+            ib = (int)(((long long)S1 * (c + 1)) / parts);
+        }
         const int b = uniform(grad_task_matrix(tprefix, nmat, t));
         const int off = moff ? uniform(moff[b]) : 0;
         const int n = moff ? uniform(moff[b + 1]) - off : n_one;
@@ -129,16 +182,18 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, kWave));
         const double pivmin = safmin * fmax(1.0, emax * emax);
         if (n == 1) {
-            if (lane == 0) {
+            if (lane == 0 && kind == kGAll) {
                 part[0] = gi;  // d lambda / d D = 1
                 part[1] = 0.0;
             }
             continue;
         }
         const int S = (n + kSeg - 1) / kSeg;  // segments; only the last may be short
-        double* __restrict__ cpF = cpbuf + (long long)wave * slot;
+        const long long slot = 3LL * (S + 1) * kWave;
+        double* __restrict__ cpF = cpbuf + (cpoff ? cpoff[b] : 0) + (long long)(t - t0) * slot;
         double* __restrict__ cpB = cpF + (long long)(S + 1) * kWave;
         double* __restrict__ zc = cpB + (long long)(S + 1) * kWave;
+        const long long tl = (long long)t * kWave + lane;
         auto at = [&](int s) { return (long long)s * kWave + lane; };
         // D and E reach the segments through a window of kWin rows in this
         // wave's LDS (refilled with coalesced loads when a segment leaves it;
@@ -189,8 +244,31 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 M[j] = x;
             }
         };
-        // pass F: D+ chain over all rows, checkpoint at segment starts
-        {
+        // gamma over segments [sa, sb), bottom up, rows high to low: the first
+        // strictly smaller |gamma| wins (the sequential scan's tie rule)
+        auto gamma_scan = [&](int sa, int sb, double& gbest, int& rbest, bool store_b) {
+            double xhi = (sb < S) ? cpB[at(sb)] : 0.0;
+            double cpn = cpF[at(sb - 1)];
+            for (int s = sb - 1; s >= sa; s--) {
+                const int lo = s * kSeg, len = min(kSeg, n - lo);
+                const double cpc = cpn;
+                if (s > sa) cpn = cpF[at(s - 1)];  // next segment's checkpoint in flight
+                load_seg(lo, false);
+                seg_dp(cpc);
+                seg_dm(len, s == S - 1, xhi);
+#pragma unroll
+                for (int j = kSeg - 1; j >= 0; j--) {
+                    const double gam = fabs(P[j] + M[j] - (dv[j] - mu));
+                    const bool better = (j < len) && gam < gbest;
+                    gbest = better ? gam : gbest;
+                    rbest = better ? lo + j : rbest;
+                }
+                if (store_b) cpB[at(s)] = M[0];
+                xhi = M[0];
+            }
+        };
+        // pass F
+        if (kind == kGAll || kind == kGChainF) {
             double x = pivot_guard(D[0] - mu, pivmin);
             for (int s = 0; s < S; s++) {
                 const int lo = s * kSeg;
@@ -200,34 +278,46 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 // D+ of row lo+16 continues the chain (P holds rows lo..lo+15)
                 x = step(ev[kSeg - 1], P[kSeg - 1], dv[kSeg]);
             }
+            if (kind == kGChainF) continue;
         }
-        // pass B: D- chain downward; twist index r = argmin |gamma_k|
-        double gbest = HUGE_VAL;
-        int r = n - 1;
-        {
+        // pass B alone (the chain; the twist search comes later)
+        if (kind == kGChainB) {
             double xhi = 0.0;
-            double cpn = cpF[at(S - 1)];
             for (int s = S - 1; s >= 0; s--) {
                 const int lo = s * kSeg, len = min(kSeg, n - lo);
-                const double cpc = cpn;
-                if (s > 0) cpn = cpF[at(s - 1)];  // next segment's checkpoint in flight
                 load_seg(lo, false);
-                seg_dp(cpc);
                 seg_dm(len, s == S - 1, xhi);
-#pragma unroll
-                for (int j = kSeg - 1; j >= 0; j--) {
-                    const double gam = fabs(P[j] + M[j] - (dv[j] - mu));
-                    const bool better = (j < len) && gam < gbest;
-                    gbest = better ? gam : gbest;
-                    r = better ? lo + j : r;
-                }
                 cpB[at(s)] = M[0];
                 xhi = M[0];
+            }
+            continue;
+        }
+        int r = n - 1;
+        if (kind == kGAll) {  // pass B with the twist search
+            double gbest = HUGE_VAL;
+            gamma_scan(0, S, gbest, r, true);
+        }
+        if (kind == kGGamma) {  // the twist search over [ia, ib)
+            double gbest = HUGE_VAL;
+            int rb = n - 1;
+            if (ia < ib) gamma_scan(ia, ib, gbest, rb, false);
+            garr[(long long)q * kWave + lane] = gbest;
+            rgarr[(long long)q * kWave + lane] = rb;
+            continue;
+        }
+        if (kind == kGZDown || kind == kGZUp) {  // the twist from the range results
+            double gbest = HUGE_VAL;
+            for (int c = ib - 1; c >= 0; c--) {
+                const long long x = (long long)(ia + c) * kWave + lane;
+                const double gv = garr[x];
+                const bool better = gv < gbest;
+                gbest = better ? gv : gbest;
+                r = better ? rgarr[x] : r;
             }
         }
         // pass Z: z with z_r = 1, |z|^2, and zc[s] = z at each segment start
         double nrm = 1.0;
-        {
+        if (kind == kGAll || kind == kGZDown) {
             // rows below r: z_k = -(E_k / D+_k) z_{k+1}, segments from the top down
             double z = 1.0;
             double cpn = cpF[at(S - 1)];
@@ -246,8 +336,16 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 }
                 if (lo <= r) zc[at(s)] = z;
             }
+            if (kind == kGZDown) {
+                rarr[tl] = r;
+                ndarr[tl] = nrm;
+                continue;
+            }
+        }
+        if (kind == kGAll || kind == kGZUp) {
+            if (kind == kGZUp) nrm = 0.0;
             // rows above r: z_k = -(E_{k-1} / D-_k) z_{k-1}, segments upward
-            z = 1.0;
+            double z = 1.0;
             double cbn = (S > 1) ? cpB[at(1)] : 0.0;
             for (int s = 0; s < S; s++) {
                 const int lo = s * kSeg, len = min(kSeg, n - lo);
@@ -266,18 +364,29 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                     if (j == 0 && lo > r) zc[at(s)] = z;
                 }
             }
+            if (kind == kGZUp) {
+                nuarr[tl] = nrm;
+                continue;
+            }
         }
-        // pass C: contributions g_i z_k^2 / |z|^2 and 2 g_i z_k z_{k+1} / |z|^2,
-        // summed over the wave's eigenvalues, one segment at a time
+        // pass C: all segments (kGAll) or [ia, ib) (kGContrib)
+        if (kind == kGContrib) {
+            r = rarr[tl];
+            nrm = ndarr[tl] + nuarr[tl];
+        }
         const double sc = active ? gi / nrm : 0.0;
-        // checkpoints of the next two segments are in flight (F: D+, B: D- of
-        // the row after the segment, Z: z at the segment start)
-        auto ldF = [&](int q) { return (q < S) ? cpF[at(q)] : 0.0; };
-        auto ldB = [&](int q) { return (q < S) ? cpB[at(q)] : 0.0; };
-        auto ldZ = [&](int q) { return (q < S) ? zc[at(q)] : 0.0; };
-        double f0 = ldF(0), f1 = ldF(1), b0 = ldB(1), b1 = ldB(2), z0 = ldZ(0), z1 = ldZ(1),
-               z2 = ldZ(2);
-        for (int s = 0; s < S; s++) {
+        const int sa = (kind == kGAll) ? 0 : ia;
+        const int sb = (kind == kGAll) ? S : ib;
+        // contributions g_i z_k^2 / |z|^2 and 2 g_i z_k z_{k+1} / |z|^2, summed
+        // over the wave's eigenvalues, one segment at a time; the checkpoints of
+        // the next two segments are in flight (F: D+, B: D- of the row after
+        // the segment, Z: z at the segment start)
+        auto ldF = [&](int x) { return (x < S) ? cpF[at(x)] : 0.0; };
+        auto ldB = [&](int x) { return (x < S) ? cpB[at(x)] : 0.0; };
+        auto ldZ = [&](int x) { return (x < S) ? zc[at(x)] : 0.0; };
+        double f0 = ldF(sa), f1 = ldF(sa + 1), b0 = ldB(sa + 1), b1 = ldB(sa + 2), z0 = ldZ(sa),
+               z1 = ldZ(sa + 1), z2 = ldZ(sa + 2);
+        for (int s = sa; s < sb; s++) {
             const int lo = s * kSeg, len = min(kSeg, n - lo);
             const double cF = f0, cB = b0, zlo = z0, zhi = z1;  // zhi: z of row lo+16
             f0 = f1;
