@@ -378,6 +378,7 @@ __global__ __launch_bounds__(kWideThreads) void k_wide_scatter(
         counts[kCountStride * merge + 0] = K;
         counts[kCountStride * merge + 2] = Tn;
         counts[kCountStride * merge + 3] = 0;
+        counts[kCountStride * merge + 4] = 0;
     }
 }
 

@@ -14,10 +14,11 @@
 //
 // Layout: one lane per eigenvalue, 64 eigenvalues of one matrix per wave (a
 // task).  Every lane walks the same row k at the same time, so D and E come
-// in as scalar loads, and the per-lane pivots live in the task's slot of two
-// scratch arrays [row][lane] (coalesced).  Each task writes its partial sums
-// for its matrix's rows; a second kernel adds the tasks of each matrix in a
-// fixed order (deterministic results).
+// through a window in the wave's LDS, and the per-lane pivots are kept only
+// at every kSeg-th row (checkpoints [segment][lane], coalesced) and recomputed
+// in registers between them.  Each task writes its partial sums for its
+// matrix's rows; a second kernel adds the tasks of each matrix in a fixed
+// order (deterministic results).
 #pragma once
 
 #include "brdc_kernels.h"

@@ -12,8 +12,8 @@
 //     r*   per root: lambda, tau, near1, near2 (+ int org)          [secular]
 //     w    product-form update vector                               [wprod]
 //     qo*  new boundary rows of the roots                           [rows]
-//   counts: kCountStride ints per merge (K, G, T, secular evaluations; the
-//           rest spare)
+//   counts: kCountStride ints per merge (K, G, T, secular evaluations,
+//           roots claimed by the streamed secular kernel; the rest spare)
 // Every kernel derives its merge geometry from (n, depth, merge index), so a
 // level is a fixed launch sequence with no host round trip.
 #pragma once
@@ -776,6 +776,7 @@ __global__ __launch_bounds__(T) void k_deflate(
         counts[kCountStride * merge + 1] = G;
         counts[kCountStride * merge + 2] = Tn;
         counts[kCountStride * merge + 3] = 0;
+        counts[kCountStride * merge + 4] = 0;
     }
 }
 
@@ -1150,24 +1151,25 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
 // Streamed secular solve for the merges with K >= kstream (the big merges
 // of the top levels, where the O(K^2) work sits).
 //
-// A workgroup of W waves serves 64/L roots at a time, L lanes per root, the
-// same root positions in every wave; the wave w streams the pole slice
-// [K*w/W, K*(w+1)/W) and lane c of a root takes every L-th pole of it.
-// Every lane of a wave needs the same few poles at the same time: the wave
-// fetches 128 poles with two coalesced loads and the lanes read them from
-// LDS (L distinct addresses per read, broadcast within each root's lanes).
-// The waves' partial sums meet in LDS; wave 0 alone holds the roots' state
-// machines, advances them and publishes the next pass's evaluation points,
-// so a pass costs two barriers and one state-machine step.  A root group
-// whose root has converged takes the next root of the workgroup's own
-// contiguous run of the merge's roots, so no lane idles while the run has
-// roots left.  L is chosen per run so that each root position works through
-// several roots (the last root's passes are a small tail of the total) while
-// each lane still streams a long run of poles per pass.
+// A workgroup of W waves serves 2 * 64/L roots at a time: L lanes per root
+// position and two roots (slots) per position, the same positions in every
+// wave.  Wave w streams the pole slice [K*w/W, K*(w+1)/W) and lane c of a
+// position takes every L-th pole of it, for both of its roots.  Every lane
+// of a wave needs the same few poles at the same time: the wave fetches 128
+// poles with two coalesced loads and the lanes read them from LDS (L
+// distinct addresses per read, broadcast within each position's lanes).
+// The LDS read bandwidth, not the arithmetic, bounds one root per lane;
+// two roots per lane halve the LDS bytes per pole evaluation.  The waves'
+// partial sums meet in LDS; wave s holds the state machines of slot s,
+// advances them and publishes the next pass's evaluation points, so a pass
+// costs two barriers and one state-machine step.
 //
-// The split of a pass's poles into the psi side (below a root: delta < 0)
-// and the phi side is per root; all roots' excluded poles lie in a window
-// [wlo, whi) of the merge, so only the poles inside it need per-lane tests.
+// Roots are handed out dynamically: a slot whose root has converged takes
+// the next root of the merge's counter.  Roots need 3 to 5 passes depending
+// on where they lie in the spectrum, so a static split leaves some
+// workgroups with far more passes than others.  L is a function of the
+// merge alone (its size and its share of the grid): a root's arithmetic
+// never depends on which workgroup or which neighbours it ran with.
 constexpr int kStreamBatch = 8;
 constexpr int kStageChunk = 128;  // poles staged per chunk (two per lane)
 
@@ -1186,6 +1188,17 @@ __device__ __forceinline__ double rcp_stream(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
+// A load that names the global address space.  Through a generic pointer the
+// compiler emits a flat load, which also counts against the LDS wait counter:
+// the first LDS read after the next chunk's prefetch would then wait for
+// that prefetch.
+typedef double vec2d __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const vec2d global_vec2d;
+__device__ __forceinline__ double2 load_global(const double2* p, int j) {
+    const vec2d v = ((global_vec2d*)p)[j];
+    return make_double2(v.x, v.y);
+}
+
 // Poles [ja, jb) pass through this wave's LDS slot in chunks of 128: lane l
 // fetches poles c0 + l and c0 + 64 + l (the next chunk's loads are in flight
 // while the current chunk is consumed).  One wave's LDS operations complete
@@ -1196,104 +1209,196 @@ __device__ __forceinline__ void staged_range(const double2* __restrict__ dz, dou
     if (ja >= jb) return;
     const int lane = threadIdx.x & (kWave - 1);
     const double2 zero = make_double2(0.0, 0.0);
-    double2 n0 = (ja + lane < jb) ? dz[ja + lane] : zero;
-    double2 n1 = (ja + kWave + lane < jb) ? dz[ja + kWave + lane] : zero;
+    double2 n0 = (ja + lane < jb) ? load_global(dz, ja + lane) : zero;
+    double2 n1 = (ja + kWave + lane < jb) ? load_global(dz, ja + kWave + lane) : zero;
     for (int c0 = ja; c0 < jb; c0 += kStageChunk) {
         slot[lane] = n0;
         slot[kWave + lane] = n1;
         const int c1 = c0 + kStageChunk;
         if (c1 < jb) {
-            n0 = (c1 + lane < jb) ? dz[c1 + lane] : zero;
-            n1 = (c1 + kWave + lane < jb) ? dz[c1 + kWave + lane] : zero;
+            n0 = (c1 + lane < jb) ? load_global(dz, c1 + lane) : zero;
+            n1 = (c1 + kWave + lane < jb) ? load_global(dz, c1 + kWave + lane) : zero;
         }
         consume(c0, min(kStageChunk, jb - c0));
     }
 }
 
-// Poles [ja, jb) that lie wholly on one side of every root; this lane (c of
-// its root's L lanes) takes poles ja + c, ja + c + L, ...  L is a template
-// parameter so the batch's LDS reads take immediate offsets.
-template <int L>
-__device__ __forceinline__ void stream_side(const double2* __restrict__ dz, double2* slot, int ja,
-                                            int jb, int c, double org, double tau,
-                                            StreamAcc& a) {
+constexpr int kStreamSlots = 1;  // roots per lane of the streamed kernel
+// Exclusion flags: one bit per 128 poles of a merge (the stage chunk), set
+// when an active root excludes a pole there; merges beyond kFlagWords * 32
+// chunks run every chunk in the excluding form.
+constexpr int kFlagWords = 128;
+
+// One wave's share [q0, q1) of a pass for this lane's roots r < NR, whose
+// excluded poles are j with (unsigned)(j - exlo[r]) < nex[r].  Lane c of a
+// position takes poles c, c + L, ... of each batch of kStreamBatch * L
+// poles; the batches lie on a grid fixed by q0.  A chunk that holds no
+// active root's excluded pole (the bulk: `flags` clear) runs the plain form.
+// A chunk that holds one runs the same arithmetic with the excluded terms
+// replaced by exact zeros, so every root that is not excluded there gets
+// bitwise the sums of the plain form: a root's sums do not depend on which
+// roots share its wave.  Each root's terms go to one running sum; the part
+// below the root (delta < 0, the psi side) is split off at the batch that
+// holds its first excluded pole, and the rest is the phi side.
+template <int L, int NR>
+__device__ __forceinline__ void stream_pass(const double2* __restrict__ dz, double2* slot, int q0,
+                                            int q1, int c, const double (&org)[NR],
+                                            const double (&tau)[NR], const int (&exlo)[NR],
+                                            const int (&nex)[NR], const unsigned* flags,
+                                            bool flag_all, StreamAcc (&lo)[NR],
+                                            StreamAcc (&hi)[NR]) {
+    // The two batch forms spell out every fused multiply-add; the compiler
+    // may fuse nothing else, so both give each root the same roundings.
+#pragma clang fp contract(off)
+    constexpr int kB = kStreamBatch * L;
     constexpr double dL = (double)L;
-    staged_range(dz, slot, ja, jb, [&](int c0, int nb) {
-        const int nfull = nb / (kStreamBatch * L);  // batches every lane has in full
+    StreamAcc cur[NR];
+    bool split[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        cur[r] = StreamAcc{0.0, 0.0, 0.0};
+        lo[r] = StreamAcc{0.0, 0.0, 0.0};
+        hi[r] = StreamAcc{0.0, 0.0, 0.0};
+        split[r] = false;
+    }
+    staged_range(dz, slot, q0, q1, [&](int c0, int nb) {
+        bool win = flag_all;
+        if (!win) {
+            const int ga = c0 / kStageChunk, gb = (c0 + nb - 1) / kStageChunk;
+            const unsigned fa = uniform((int)flags[ga >> 5]), fb = uniform((int)flags[gb >> 5]);
+            win = (((fa >> (ga & 31)) | (fb >> (gb & 31))) & 1u) != 0;
+        }
+        const int nfull = nb / kB;  // batches every lane has in full
         int o = c;
-        for (int k = 0; k < nfull; k++, o += kStreamBatch * L) {
-            double2 p[kStreamBatch];
+        if (!win) {
+            for (int bi = 0; bi < nfull; bi++, o += kB) {
+                double2 p[kStreamBatch];
 #pragma unroll
-            for (int b = 0; b < kStreamBatch; b++) p[b] = slot[o + b * L];
-            // J gets sum_b (j + b L) t_b = j * T + L * sum_b b t_b
-            double T = 0.0, B = 0.0;
+                for (int b = 0; b < kStreamBatch; b++) p[b] = slot[o + b * L];
+                const int jl = c0 + o;  // this lane's first pole of the batch
 #pragma unroll
-            for (int b = 0; b < kStreamBatch; b += 4) {
-                // one reciprocal for four poles: with u = xy, v = zw,
-                // 1/u = v/(uv) and 1/v = u/(uv), then 1/x = y/u and so on.
-                // Poles closer than the deflation tolerance were deflated, so
-                // every delta here is at least about 1e-16 and at most the
-                // spectrum's width: uv neither overflows nor underflows.
-                double dl[4];
+                for (int r = 0; r < NR; r++) {
+                    // J gets sum_b (jl + b L) t_b = jl * T + L * sum_b b t_b
+                    double T = 0.0, B = 0.0;
 #pragma unroll
-                for (int q = 0; q < 4; q++) dl[q] = (p[b + q].x - org) - tau;
-                const double u = dl[0] * dl[1], v = dl[2] * dl[3];
-                const double ruv = rcp_stream(u * v);
-                const double ru = v * ruv, rv = u * ruv;
-                const double r[4] = {dl[1] * ru, dl[0] * ru, dl[3] * rv, dl[2] * rv};
+                    for (int b = 0; b < kStreamBatch; b += 4) {
+                        // one reciprocal for four poles: with u = xy, v = zw,
+                        // 1/u = v/(uv) and 1/v = u/(uv), then 1/x = y/u and so
+                        // on.  Poles closer than the deflation tolerance were
+                        // deflated, so every delta here is at least about
+                        // 1e-16 and at most the spectrum's width: uv neither
+                        // overflows nor underflows.
+                        double dl[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const double t = p[b + q].y * r[q];
-                    T += t;
-                    a.D = fma(t, r[q], a.D);
-                    if (b + q > 0) B = fma((double)(b + q), t, B);
+                        for (int q = 0; q < 4; q++) dl[q] = (p[b + q].x - org[r]) - tau[r];
+                        const double u = dl[0] * dl[1], v = dl[2] * dl[3];
+                        const double ruv = rcp_stream(u * v);
+                        const double ru = v * ruv, rv = u * ruv;
+                        const double rr[4] = {dl[1] * ru, dl[0] * ru, dl[3] * rv, dl[2] * rv};
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const double t = p[b + q].y * rr[q];
+                            T = fma(p[b + q].y, rr[q], T);
+                            cur[r].D = fma(t, rr[q], cur[r].D);
+                            if (b + q > 0) B = fma((double)(b + q), t, B);
+                        }
+                    }
+                    cur[r].S += T;
+                    cur[r].J = fma((double)jl, T, cur[r].J);
+                    cur[r].J = fma(dL, B, cur[r].J);
                 }
             }
-            a.S += T;
-            a.J = fma((double)(c0 + o), T, a.J);
-            a.J = fma(dL, B, a.J);
+        } else {
+            for (int bi = 0; bi < nfull; bi++, o += kB) {
+                double2 p[kStreamBatch];
+#pragma unroll
+                for (int b = 0; b < kStreamBatch; b++) p[b] = slot[o + b * L];
+                const int j0 = c0 + bi * kB, jl = c0 + o;
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    // the root's own batch splits its terms at exlo: Tp, Dp, Bp
+                    // collect the part above it
+                    const bool at = (unsigned)(exlo[r] - j0) < (unsigned)kB;
+                    double T = 0.0, B = 0.0, Tp = 0.0, Dp = 0.0, Bp = 0.0;
+#pragma unroll
+                    for (int b = 0; b < kStreamBatch; b += 4) {
+                        double dl[4];
+                        bool ex[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            ex[q] = (unsigned)(jl + (b + q) * L - exlo[r]) < (unsigned)nex[r];
+                            dl[q] = ex[q] ? 1.0 : (p[b + q].x - org[r]) - tau[r];
+                        }
+                        const double u = dl[0] * dl[1], v = dl[2] * dl[3];
+                        const double ruv = rcp_stream(u * v);
+                        const double ru = v * ruv, rv = u * ruv;
+                        const double rr[4] = {dl[1] * ru, dl[0] * ru, dl[3] * rv, dl[2] * rv};
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const double y = ex[q] ? 0.0 : p[b + q].y;
+                            const bool above = at && (jl + (b + q) * L > exlo[r]);
+                            const double yc = above ? 0.0 : y, ya = above ? y : 0.0;
+                            const double tc = yc * rr[q], ta = ya * rr[q];
+                            T = fma(yc, rr[q], T);
+                            cur[r].D = fma(tc, rr[q], cur[r].D);
+                            Tp = fma(ya, rr[q], Tp);
+                            Dp = fma(ta, rr[q], Dp);
+                            if (b + q > 0) {
+                                B = fma((double)(b + q), tc, B);
+                                Bp = fma((double)(b + q), ta, Bp);
+                            }
+                        }
+                    }
+                    cur[r].S += T;
+                    cur[r].J = fma((double)jl, T, cur[r].J);
+                    cur[r].J = fma(dL, B, cur[r].J);
+                    if (at) {
+                        lo[r] = cur[r];
+                        cur[r].S = Tp;
+                        cur[r].D = Dp;
+                        cur[r].J = fma(dL, Bp, (double)jl * Tp);
+                        split[r] = true;
+                    }
+                }
+            }
         }
+        // the end of the share: fewer than kB poles, one at a time
         for (; o < nb; o += L) {
-            const double2 p = slot[o];
-            const double r = rcp_stream((p.x - org) - tau);
-            const double t = p.y * r;
-            a.S += t;
-            a.D = fma(t, r, a.D);
-            a.J = fma((double)(c0 + o), t, a.J);
-        }
-    });
-}
-
-// Poles [ja, jb) inside the window: each root skips its excluded poles
-// [exlo, exlo + nex) and files the others by side.
-__device__ __forceinline__ void stream_window(const double2* __restrict__ dz, double2* slot,
-                                              int ja, int jb, int c, int L, double org,
-                                              double tau, int exlo, int nex, StreamAcc& lo,
-                                              StreamAcc& hi) {
-    staged_range(dz, slot, ja, jb, [&](int c0, int nb) {
-        for (int o = c; o < nb; o += L) {
             const int j = c0 + o;
             const double2 p = slot[o];
-            if ((unsigned)(j - exlo) < (unsigned)nex) continue;
-            const double r = rcp_nr((p.x - org) - tau);
-            const double t = p.y * r;
-            StreamAcc& a = (j < exlo) ? lo : hi;
-            a.S += t;
-            a.D = fma(t, r, a.D);
-            a.J = fma((double)j, t, a.J);
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                if (!split[r] && j >= exlo[r]) {
+                    lo[r] = cur[r];
+                    cur[r] = StreamAcc{0.0, 0.0, 0.0};
+                    split[r] = true;
+                }
+                const bool ex = (unsigned)(j - exlo[r]) < (unsigned)nex[r];
+                const double rr = rcp_stream(ex ? 1.0 : (p.x - org[r]) - tau[r]);
+                const double t = ex ? 0.0 : p.y * rr;
+                cur[r].S += t;
+                cur[r].D = fma(t, rr, cur[r].D);
+                cur[r].J = fma((double)j, t, cur[r].J);
+            }
         }
     });
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        // unsplit with the root above the share: every pole lies below it
+        const bool below = !split[r] && exlo[r] >= q0;
+        lo[r].S = below ? cur[r].S : lo[r].S;
+        lo[r].D = below ? cur[r].D : lo[r].D;
+        lo[r].J = below ? cur[r].J : lo[r].J;
+        hi[r].S = below ? 0.0 : cur[r].S;
+        hi[r].D = below ? 0.0 : cur[r].D;
+        hi[r].J = below ? 0.0 : cur[r].J;
+    }
 }
 
-__device__ __forceinline__ int wave_max(int x) {
-    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, kWave));
-    return x;
-}
-
-// Lanes per root of the streamed kernel for a workgroup's run of `roots`
-// roots of a K-pole merge: enough lanes that each of the 64 / L root
-// positions works through about 4 or more roots, no more than keeps 128 poles
-// per lane per pass, at most 16.
+// Lanes per root position of the streamed kernel for a K-pole merge whose
+// workgroups take about `roots` roots per slot each: enough lanes that each
+// of the 64 / L positions works through about 4 or more roots per slot, no
+// more than keeps 128 poles per lane per pass, at most 16.
 __device__ __forceinline__ int stream_lanes(int K, int roots, int W) {
     const int want = (256 + roots - 1) / max(roots, 1);
     int L = 1;
@@ -1302,43 +1407,52 @@ __device__ __forceinline__ int stream_lanes(int K, int roots, int W) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kWave * W) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ __launch_bounds__(kWave * W)
+__attribute__((amdgpu_waves_per_eu(kStreamSlots == 1 ? 4 : 2, kStreamSlots == 1 ? 4 : 2)))
 void k_secular_stream(
     Geo g, int merges, int kstream, const double* __restrict__ ew, const int* counts,
     const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
     double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw,
     const int* __restrict__ moff, int* info) {
-    __shared__ double part[W][6][kWave];
+    static_assert(W >= kStreamSlots, "one advancing wave per slot");
+    constexpr int NR = kStreamSlots;
+    __shared__ double part[W][NR][6][kWave];
     __shared__ double2 stage[W][kStageChunk];
-    __shared__ double pass_org[kWave], pass_tau[kWave];
-    __shared__ int pass_exlo[kWave], pass_nex[kWave];
-    __shared__ int pass_live;
+    __shared__ double pass_org[NR][kWave], pass_tau[NR][kWave];
+    __shared__ int pass_exlo[NR][kWave], pass_nex[NR][kWave];
+    __shared__ unsigned pass_flags[2][kFlagWords];  // by pass parity
+    __shared__ int pass_live[NR];
+    // the slots' root state machines, one per lane and slot: kept in LDS, so
+    // no registers hold them through the pole loops
+    __shared__ SecularSolve<PoleView> roots[NR][kWave];
     const int wave = uniform(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     double2* slot = stage[wave];
-    // Static split of the level's streamed roots: a root of a K-pole merge
-    // weighs K (its passes each stream K poles); workgroup b owns the roots
-    // whose weight interval starts in [W b / G, W (b+1) / G), a contiguous
-    // run of each merge, so its live roots -- and the exclusion window of a
-    // pass -- stay close together.
+    const bool owner = wave < NR;  // wave s advances slot s
+    // Workgroups to merges: a root of a K-pole merge weighs K (its passes each
+    // stream K poles), and workgroup b covers the weight interval
+    // [wsum b / G, wsum (b+1) / G); it serves, in order, every merge whose
+    // interval meets its own, taking roots there until the merge's counter
+    // runs out.
     long long wsum = 0;
     for (int m = 0; m < merges; m++) {
         const long long K = uniform(counts[kCountStride * m]);
         if (K >= kstream) wsum += K * K;
     }
-    const long long wa = wsum * blockIdx.x / gridDim.x;
-    const long long wb = wsum * (blockIdx.x + 1) / gridDim.x;
+    const long long G = gridDim.x;
+    const long long wa = wsum * blockIdx.x / G;
+    const long long wb = wsum * (blockIdx.x + 1) / G;
     long long wbase = 0;
     for (int merge = 0; merge < merges; merge++) {
         const int K = uniform(counts[kCountStride * merge]);
         if (K < kstream) continue;
-        const long long w0 = wbase;
-        wbase += (long long)K * K;
-        // roots r with w0 + r K in [wa, wb)
-        auto first_root = [&](long long x) {
-            return (int)min<long long>(K, x <= w0 ? 0 : (x - w0 + K - 1) / K);
-        };
-        const int r0 = first_root(wa), r1 = first_root(wb);
-        if (r0 >= r1) continue;
+        const long long w0 = wbase, w1 = wbase + (long long)K * K;
+        wbase = w1;
+        if (w1 <= wa || w0 >= wb) continue;
+        // the workgroups that serve this merge: b with [wsum b/G, wsum (b+1)/G)
+        // meeting [w0, w1)
+        const long long bfirst = w0 * G / wsum;
+        const long long blast = (w1 * G + wsum - 1) / wsum - 1;
+        const int share = (int)max<long long>(1, K / max<long long>(1, blast - bfirst + 1));
         const Item it = g.item(merge);
         const int s = it.s, sz = it.sz;
         const FailSite fs = fail_site(info, moff, g.n, it);
@@ -1346,47 +1460,72 @@ void k_secular_stream(
         const double2* __restrict__ dz = adz + s;
         const PoleView E{dz};
         const int q0 = (int)(((long long)K * wave) / W), q1 = (int)(((long long)K * (wave + 1)) / W);
-        const int L = uniform(stream_lanes(K, r1 - r0, W));
-        const int c = lane & (L - 1), lead = lane - c;  // lane c of the root at lanes lead..
-        // wave 0 owns the roots: state machine, claims, results
-        SecularSolve<PoleView> st;
-        int i = -1;  // this lane's root, -1 when it needs one
-        bool drained = false;
-        int next = r0;  // next unclaimed root of this workgroup's run
+        const int L = uniform(stream_lanes(K, max(1, share / NR), W));
+        const int c = lane & (L - 1), lead = lane - c;  // lane c of the position at lanes lead..
+        const bool flag_all = K > kFlagWords * 32 * kStageChunk;
+        int* claim = counts_rw + kCountStride * merge + 4;
+        for (int x = threadIdx.x; x < 2 * kFlagWords; x += kWave * W) pass_flags[x / kFlagWords][x % kFlagWords] = 0u;
+        __syncthreads();
+        // an owning wave's slot: state machine, claims, results
+        SecularSolve<PoleView>& st = roots[owner ? wave : 0][lane];
+        int i = -1;              // this lane's root in the slot, -1 when it needs one
+        bool exhausted = false;  // the merge's counter has run out
         int iters = 0;
+        int parity = 0;
         for (;;) {
-            if (wave == 0) {
-                if (!drained && __any(i < 0)) {
-                    // one claim per root group (its lane c = 0), numbered by rank
-                    const unsigned long long need = __ballot(i < 0 && c == 0);
-                    const int f = next + (int)__popcll(need & ((1ull << lead) - 1ull));
-                    next += (int)__popcll(need);
-                    if (i < 0 && f < r1) {
-                        i = f;
+            if (owner) {
+                const unsigned long long need = __ballot(i < 0 && c == 0);
+                if (need && !exhausted) {
+                    // one root per needing position, numbered by rank
+                    const int nneed = (int)__popcll(need);
+                    const int rank = (int)__popcll(need & ((1ull << lead) - 1ull));
+                    int b0 = 0;
+                    if (lane == 0) b0 = atomicAdd(claim, nneed);
+                    const int base = uniform(__shfl(b0, 0, kWave));
+                    if (base + nneed >= K) exhausted = true;
+                    if (i < 0 && base + rank < K) {
+                        i = base + rank;
                         st.init(E, K, i, rho);
                     }
-                    drained = __any(i < 0 && f >= r1);  // the run is handed out
                 }
                 const bool active = (i >= 0);
-                pass_org[lane] = active ? st.ev_org : 0.0;
-                pass_tau[lane] = active ? st.ev_tau : 0.0;
-                pass_exlo[lane] = active ? st.ev_exlo : K;
-                pass_nex[lane] = active ? st.ev_nex : 0;
+                const int exlo = active ? st.ev_exlo : K, nex = active ? st.ev_nex : 0;
+                pass_org[wave][lane] = active ? st.ev_org : 0.0;
+                pass_tau[wave][lane] = active ? st.ev_tau : 0.0;
+                pass_exlo[wave][lane] = exlo;
+                pass_nex[wave][lane] = nex;
+                if (active && c == 0 && !flag_all) {
+                    const int ga = exlo / kStageChunk, gb = (exlo + nex - 1) / kStageChunk;
+                    atomicOr(&pass_flags[parity][ga >> 5], 1u << (ga & 31));
+                    atomicOr(&pass_flags[parity][gb >> 5], 1u << (gb & 31));
+                }
                 const bool live = __any(active);
-                if (lane == 0) pass_live = live ? 1 : 0;
+                if (lane == 0) pass_live[wave] = live ? 1 : 0;
             }
             __syncthreads();
-            if (!pass_live) break;  // the merge is done (same value in every wave)
-            const double org = pass_org[lane], tau = pass_tau[lane];
-            const int exlo = pass_exlo[lane], nex = pass_nex[lane];
-            const int wlo = uniform(wave_min(exlo)), whi = uniform(wave_max(nex > 0 ? exlo + nex : 0));
-            StreamAcc lo = {0.0, 0.0, 0.0}, hi = {0.0, 0.0, 0.0};
+            int nlive = 0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) nlive += pass_live[r];
+            if (uniform(nlive) == 0) break;  // the merge is done (same value in every wave)
+            if (wave == W - 1) {
+                // the other parity's flags were read in the last pass and are
+                // set again only after this pass's second barrier
+                for (int x = lane; x < kFlagWords; x += kWave) pass_flags[parity ^ 1][x] = 0u;
+            }
+            double org[NR], tau[NR];
+            int exl[NR], nx[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                org[r] = pass_org[r][lane];
+                tau[r] = pass_tau[r][lane];
+                exl[r] = pass_exlo[r][lane];
+                nx[r] = pass_nex[r][lane];
+            }
+            StreamAcc lo[NR], hi[NR];
             auto pass = [&](auto lanes) {
                 constexpr int LL = decltype(lanes)::value;
-                stream_side<LL>(dz, slot, q0, min(q1, wlo), c, org, tau, lo);
-                stream_window(dz, slot, max(q0, wlo), min(q1, whi), c, LL, org, tau, exlo, nex,
-                              lo, hi);
-                stream_side<LL>(dz, slot, max(q0, whi), q1, c, org, tau, hi);
+                stream_pass<LL, NR>(dz, slot, q0, q1, c, org, tau, exl, nx, pass_flags[parity],
+                                    flag_all, lo, hi);
             };
             switch (L) {  // stream_lanes returns 1, 2, 4, 8 or 16
                 case 1: pass(std::integral_constant<int, 1>{}); break;
@@ -1395,22 +1534,24 @@ void k_secular_stream(
                 case 8: pass(std::integral_constant<int, 8>{}); break;
                 default: pass(std::integral_constant<int, 16>{}); break;
             }
-            double acc[6] = {lo.S, lo.D, lo.J, hi.S, hi.D, hi.J};
-            for (int o = 1; o < L; o <<= 1) {
 #pragma unroll
-                for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+            for (int r = 0; r < NR; r++) {
+                double acc[6] = {lo[r].S, lo[r].D, lo[r].J, hi[r].S, hi[r].D, hi[r].J};
+                for (int o = 1; o < L; o <<= 1) {
+#pragma unroll
+                    for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+                }
+#pragma unroll
+                for (int q = 0; q < 6; q++) part[wave][r][q][lane] = acc[q];
             }
-#pragma unroll
-            for (int q = 0; q < 6; q++) part[wave][q][lane] = acc[q];
             __syncthreads();
             // the other waves wait at the next barrier; part[] and pass_*[] are
             // rewritten only after it
-            if (wave == 0 && i >= 0) {
-#pragma unroll
-                for (int q = 0; q < 6; q++) acc[q] = 0.0;
+            if (owner && i >= 0) {
+                double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                 for (int w = 0; w < W; w++) {
 #pragma unroll
-                    for (int q = 0; q < 6; q++) acc[q] += part[w][q][lane];
+                    for (int q = 0; q < 6; q++) acc[q] += part[w][wave][q][lane];
                 }
                 SecSums sums;
                 sums.S = acc[0] + acc[3];
@@ -1431,8 +1572,9 @@ void k_secular_stream(
                     i = -1;
                 }
             }
+            parity ^= 1;
         }
-        if (wave == 0) {
+        if (owner) {
             for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
             if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
         }

@@ -175,6 +175,43 @@ def test_graph_replay_matches_eager(gpu):
         assert torch.isfinite(x)
 
 
+def test_streamed_merges_are_reproducible(gpu, orc):
+    """Merges of 2048 poles or more go through the streamed secular kernel,
+    whose workgroups take roots from a shared counter in whatever order they
+    arrive.  A root's arithmetic does not depend on that order or on which
+    roots share its wave, so repeated eager solves and graph replays agree
+    bit for bit.  Cases: Toeplitz n = 16384 (closed form) and an evenly
+    spread diagonal with a constant coupling, n = 6000 (oracle); both keep
+    K >= 2048 at their top merges."""
+    import brdc
+    torch = gpu
+    dev = torch.device("cuda:0")
+    n1, n2 = 16384, 6000
+    cases = [(np.full(n1, 2.0), np.full(n1 - 1, 1.0),
+              np.sort(2.0 + 2.0 * np.cos(np.arange(1, n1 + 1) * np.pi / (n1 + 1)))),
+             (np.linspace(-1.0, 1.0, n2), np.full(n2 - 1, 0.3), None)]
+    for d, e, ref in cases:
+        n = d.shape[0]
+        if ref is None:
+            ref, info_ref = orc.dstebr(d, e)
+            assert info_ref == 0
+        w0, info, s = solve(torch, d, e)
+        assert info == 0 and rel_err(w0, ref) <= TOL
+        assert int(s.merge_stats()["k"].max()) >= 2048  # the streamed kernel ran
+        for _ in range(2):
+            w1, _, _ = solve(torch, d, e)
+            np.testing.assert_array_equal(w1, w0)
+        dt = torch.from_numpy(d).to(dev)
+        et = torch.from_numpy(e).to(dev)
+        gs = brdc.GraphSolve(brdc.Solver(n, device=dev), dt, et)
+        for _ in range(3):
+            gs.w.fill_(-5.0)
+            torch.cuda.synchronize()
+            out = gs.run()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(out.cpu().numpy(), w0)
+
+
 def test_matches_rocsolver_on_the_same_gpu(gpu):
     """rocSOLVER's STERF and STEDC (values only) as an independent GPU
     reference at a size they finish quickly."""
