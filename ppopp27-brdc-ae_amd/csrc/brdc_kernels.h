@@ -1151,18 +1151,20 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
 // Streamed secular solve for the merges with K >= kstream (the big merges
 // of the top levels, where the O(K^2) work sits).
 //
-// A workgroup of W waves serves 2 * 64/L roots at a time: L lanes per root
-// position and two roots (slots) per position, the same positions in every
-// wave.  Wave w streams the pole slice [K*w/W, K*(w+1)/W) and lane c of a
-// position takes every L-th pole of it, for both of its roots.  Every lane
-// of a wave needs the same few poles at the same time: the wave fetches 128
-// poles with two coalesced loads and the lanes read them from LDS (L
-// distinct addresses per read, broadcast within each position's lanes).
-// The LDS read bandwidth, not the arithmetic, bounds one root per lane;
-// two roots per lane halve the LDS bytes per pole evaluation.  The waves'
-// partial sums meet in LDS; wave s holds the state machines of slot s,
-// advances them and publishes the next pass's evaluation points, so a pass
-// costs two barriers and one state-machine step.
+// A workgroup of W waves serves kStreamSlots * 64/L roots at a time: L lanes
+// per root position and kStreamSlots roots (slots) per position, the same
+// positions in every wave.  A pass cuts the K poles into kStreamSlices fixed
+// slices; the waves take slices from a counter in LDS (the older waves issue
+// first and finish their share sooner, so a fixed slice per wave leaves them
+// idle at the barrier), and lane c of a position takes every L-th pole of a
+// slice.  Every lane of a wave needs the same few poles at the same time:
+// the wave fetches 128 poles with two coalesced loads and the lanes read
+// them from LDS (L distinct addresses per read, broadcast within each
+// position's lanes).  Each slice's partial sums have their own row in LDS
+// and are added in slice order, whichever wave computed them.  Wave s holds
+// the state machines of slot s, advances them and publishes the next pass's
+// evaluation points, so a pass costs two barriers and one state-machine
+// step.
 //
 // Roots are handed out dynamically: a slot whose root has converged takes
 // the next root of the merge's counter.  Roots need 3 to 5 passes depending
@@ -1223,7 +1225,8 @@ __device__ __forceinline__ void staged_range(const double2* __restrict__ dz, dou
     }
 }
 
-constexpr int kStreamSlots = 1;  // roots per lane of the streamed kernel
+constexpr int kStreamSlots = 1;    // roots per lane of the streamed kernel
+constexpr int kStreamSlices = 12;  // pole slices per pass of the streamed kernel
 // Exclusion flags: one bit per 128 poles of a merge (the stage chunk), set
 // when an active root excludes a pole there; merges beyond kFlagWords * 32
 // chunks run every chunk in the excluding form.
@@ -1398,11 +1401,12 @@ __device__ __forceinline__ void stream_pass(const double2* __restrict__ dz, doub
 // Lanes per root position of the streamed kernel for a K-pole merge whose
 // workgroups take about `roots` roots per slot each: enough lanes that each
 // of the 64 / L positions works through about 4 or more roots per slot, no
-// more than keeps 128 poles per lane per pass, at most 16.
-__device__ __forceinline__ int stream_lanes(int K, int roots, int W) {
+// more than leaves each lane at least 256 poles of each of the `slices`
+// slices of a pass, at most 16.
+__device__ __forceinline__ int stream_lanes(int K, int roots, int slices) {
     const int want = (256 + roots - 1) / max(roots, 1);
     int L = 1;
-    while (L < 16 && L < want && (long long)K >= 2LL * W * 128 * L) L <<= 1;
+    while (L < 16 && L < want && (long long)K >= 2LL * slices * 128 * L) L <<= 1;
     return L;
 }
 
@@ -1416,12 +1420,13 @@ void k_secular_stream(
     const int* __restrict__ moff, int* info) {
     static_assert(W >= kStreamSlots, "one advancing wave per slot");
     constexpr int NR = kStreamSlots;
-    __shared__ double part[W][NR][6][kWave];
+    __shared__ double part[kStreamSlices][NR][6][kWave];
     __shared__ double2 stage[W][kStageChunk];
     __shared__ double pass_org[NR][kWave], pass_tau[NR][kWave];
     __shared__ int pass_exlo[NR][kWave], pass_nex[NR][kWave];
     __shared__ unsigned pass_flags[2][kFlagWords];  // by pass parity
     __shared__ int pass_live[NR];
+    __shared__ int pass_next;  // the next slice to take
     // the slots' root state machines, one per lane and slot: kept in LDS, so
     // no registers hold them through the pole loops
     __shared__ SecularSolve<PoleView> roots[NR][kWave];
@@ -1459,8 +1464,7 @@ void k_secular_stream(
         const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
         const double2* __restrict__ dz = adz + s;
         const PoleView E{dz};
-        const int q0 = (int)(((long long)K * wave) / W), q1 = (int)(((long long)K * (wave + 1)) / W);
-        const int L = uniform(stream_lanes(K, max(1, share / NR), W));
+        const int L = uniform(stream_lanes(K, max(1, share / NR), kStreamSlices));
         const int c = lane & (L - 1), lead = lane - c;  // lane c of the position at lanes lead..
         const bool flag_all = K > kFlagWords * 32 * kStageChunk;
         int* claim = counts_rw + kCountStride * merge + 4;
@@ -1501,6 +1505,7 @@ void k_secular_stream(
                 }
                 const bool live = __any(active);
                 if (lane == 0) pass_live[wave] = live ? 1 : 0;
+                if (wave == 0 && lane == 0) pass_next = 0;
             }
             __syncthreads();
             int nlive = 0;
@@ -1521,37 +1526,45 @@ void k_secular_stream(
                 exl[r] = pass_exlo[r][lane];
                 nx[r] = pass_nex[r][lane];
             }
-            StreamAcc lo[NR], hi[NR];
-            auto pass = [&](auto lanes) {
-                constexpr int LL = decltype(lanes)::value;
-                stream_pass<LL, NR>(dz, slot, q0, q1, c, org, tau, exl, nx, pass_flags[parity],
-                                    flag_all, lo, hi);
-            };
-            switch (L) {  // stream_lanes returns 1, 2, 4, 8 or 16
-                case 1: pass(std::integral_constant<int, 1>{}); break;
-                case 2: pass(std::integral_constant<int, 2>{}); break;
-                case 4: pass(std::integral_constant<int, 4>{}); break;
-                case 8: pass(std::integral_constant<int, 8>{}); break;
-                default: pass(std::integral_constant<int, 16>{}); break;
-            }
-#pragma unroll
-            for (int r = 0; r < NR; r++) {
-                double acc[6] = {lo[r].S, lo[r].D, lo[r].J, hi[r].S, hi[r].D, hi[r].J};
-                for (int o = 1; o < L; o <<= 1) {
-#pragma unroll
-                    for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+            for (;;) {
+                int k0 = 0;
+                if (lane == 0) k0 = atomicAdd(&pass_next, 1);
+                const int k = uniform(__shfl(k0, 0, kWave));
+                if (k >= kStreamSlices) break;
+                const int q0 = (int)(((long long)K * k) / kStreamSlices);
+                const int q1 = (int)(((long long)K * (k + 1)) / kStreamSlices);
+                StreamAcc lo[NR], hi[NR];
+                auto pass = [&](auto lanes) {
+                    constexpr int LL = decltype(lanes)::value;
+                    stream_pass<LL, NR>(dz, slot, q0, q1, c, org, tau, exl, nx,
+                                        pass_flags[parity], flag_all, lo, hi);
+                };
+                switch (L) {  // stream_lanes returns 1, 2, 4, 8 or 16
+                    case 1: pass(std::integral_constant<int, 1>{}); break;
+                    case 2: pass(std::integral_constant<int, 2>{}); break;
+                    case 4: pass(std::integral_constant<int, 4>{}); break;
+                    case 8: pass(std::integral_constant<int, 8>{}); break;
+                    default: pass(std::integral_constant<int, 16>{}); break;
                 }
 #pragma unroll
-                for (int q = 0; q < 6; q++) part[wave][r][q][lane] = acc[q];
+                for (int r = 0; r < NR; r++) {
+                    double acc[6] = {lo[r].S, lo[r].D, lo[r].J, hi[r].S, hi[r].D, hi[r].J};
+                    for (int o = 1; o < L; o <<= 1) {
+#pragma unroll
+                        for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 6; q++) part[k][r][q][lane] = acc[q];
+                }
             }
             __syncthreads();
             // the other waves wait at the next barrier; part[] and pass_*[] are
             // rewritten only after it
             if (owner && i >= 0) {
                 double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                for (int w = 0; w < W; w++) {
+                for (int k = 0; k < kStreamSlices; k++) {
 #pragma unroll
-                    for (int q = 0; q < 6; q++) acc[q] += part[w][wave][q][lane];
+                    for (int q = 0; q < 6; q++) acc[q] += part[k][wave][q][lane];
                 }
                 SecSums sums;
                 sums.S = acc[0] + acc[3];
