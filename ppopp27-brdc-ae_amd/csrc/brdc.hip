@@ -39,6 +39,7 @@ constexpr int kPackMinMerges = 4096;
 constexpr int kSecMaxWaves = 3072;  // secular grid cap: about 3 resident waves per SIMD
 constexpr int kStreamWaves = 8;     // waves per workgroup of the streamed secular kernel
 constexpr int kStreamMinK = 2048;   // merges with at least this many poles are streamed
+constexpr int kMidMaxGroups = 1024; // midpoint-pass grid cap (tiles of 64 roots)
 
 // Workgroups of the streamed secular kernel that fit the device at once
 // (CUs x resident groups per CU), cached per device.
@@ -400,6 +401,11 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
         BRDC_LAUNCH_CHECK();
         if (streamed) {
             ProfScope ps(kSecularStream, st);
+            hipLaunchKernelGGL(k_secular_mid<kStreamWaves>,
+                               dim3(std::min(kMidMaxGroups, (n + kWave - 1) / kWave)),
+                               dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
+                               (const double*)ew, (const int*)counts, (const double2*)adz, rlam);
+            BRDC_LAUNCH_CHECK();
             const int groups = std::min(stream_groups(), (n + kWave - 1) / kWave);
             hipLaunchKernelGGL(k_secular_stream<kStreamWaves>, dim3(groups),
                                dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,

@@ -1410,6 +1410,112 @@ __device__ __forceinline__ int stream_lanes(int K, int roots, int slices) {
     return L;
 }
 
+// The first evaluation of every root of the streamed merges (DLAED4's
+// midpoint pass): S_i = sum over j outside the root's two interval poles of
+// z_j^2 / ((d_j - d_org) - tau) at tau = the interval midpoint (root i < K-1:
+// org = i, tau = (d_{i+1} - d_i)/2, poles i, i+1 excluded; root K-1: org =
+// K-1, tau = rho/2, poles K-2, K-1 excluded), written to rlam[i] for the
+// streamed kernel's initial guess.  Only S is needed here, so four terms
+// share one reciprocal and one accumulation:
+//   t0 + t1 + t2 + t3 = ((y0 x1 + y1 x0) v + (y2 x3 + y3 x2) u) / (u v),
+//   u = x0 x1, v = x2 x3, y = z^2, x = delta.
+// A tile is 64 consecutive roots, one per lane; wave w of the workgroup
+// takes the pole slice [K w/W, K (w+1)/W), the same poles for every lane
+// (wave-uniform loads).  Poles below the tile's window [i0 - 1, i0 + 65) lie
+// below every root of the tile and poles above it above, so four terms of
+// a quad share a sign there; the window's poles are taken one at a time
+// with the per-lane exclusion.  Each slice's sums meet in LDS and are added
+// in slice order.
+template <int W>
+__global__ __launch_bounds__(kWave * W) void k_secular_mid(Geo g, int merges, int kstream,
+                                                           const double* __restrict__ ew,
+                                                           const int* __restrict__ counts,
+                                                           const double2* __restrict__ adz,
+                                                           double* __restrict__ rlam) {
+    __shared__ double part[W][kWave];
+    const int wave = uniform(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
+    int t = blockIdx.x;  // this workgroup's next tile
+    int tbase = 0;       // tiles of the streamed merges before `merge`
+    for (int merge = 0; merge < merges; merge++) {
+        const int K = uniform(counts[kCountStride * merge]);
+        if (K < kstream) continue;
+        const int nt = (K + kWave - 1) / kWave;
+        for (; t < tbase + nt; t += gridDim.x) {
+            const Item it = g.item(merge);
+            const int s = it.s, sz = it.sz;
+            const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
+            const double2* __restrict__ dz = adz + s;
+            const int i0 = (t - tbase) * kWave, i = i0 + lane;
+            // the evaluation point of SecularSolve::init
+            double org = 0.0, tau = 0.0;
+            int exlo = K;
+            if (i < K - 1) {
+                org = dz[i].x;
+                tau = (dz[i + 1].x - org) * 0.5;
+                exlo = i;
+            } else if (i == K - 1) {
+                org = dz[K - 1].x;
+                tau = rho * 0.5;
+                exlo = K - 2;
+            }
+            const int wlo = max(0, i0 - 1), whi = min(K, i0 + kWave + 1);
+            const int q0 = (int)(((long long)K * wave) / W);
+            const int q1 = (int)(((long long)K * (wave + 1)) / W);
+            double S = 0.0;
+            auto plain = [&](int ja, int jb) {
+                int j = ja;
+                if (j + 8 <= jb) {
+                    // eight poles a step; the next eight are in flight meanwhile
+                    double2 nx[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) nx[q] = dz[j + q];
+                    for (; j + 8 <= jb; j += 8) {
+                        double2 p[8];
+#pragma unroll
+                        for (int q = 0; q < 8; q++) p[q] = nx[q];
+                        if (j + 16 <= jb) {
+#pragma unroll
+                            for (int q = 0; q < 8; q++) nx[q] = dz[j + 8 + q];
+                        }
+#pragma unroll
+                        for (int h = 0; h < 8; h += 4) {
+                            double x[4];
+#pragma unroll
+                            for (int q = 0; q < 4; q++) x[q] = (p[h + q].x - org) - tau;
+                            const double u = x[0] * x[1], v = x[2] * x[3];
+                            const double ruv = rcp_stream(u * v);
+                            const double a = fma(p[h + 1].y, x[0], p[h].y * x[1]);
+                            const double b = fma(p[h + 3].y, x[2], p[h + 2].y * x[3]);
+                            S = fma(fma(b, u, a * v), ruv, S);
+                        }
+                    }
+                }
+                for (; j < jb; j++) {
+                    const double2 p = dz[j];
+                    S = fma(p.y, rcp_stream((p.x - org) - tau), S);
+                }
+            };
+            plain(q0, min(q1, wlo));
+            for (int j = max(q0, wlo); j < min(q1, whi); j++) {
+                const double2 p = dz[j];
+                const bool ex = (unsigned)(j - exlo) < 2u;
+                const double r = rcp_stream(ex ? 1.0 : (p.x - org) - tau);
+                S = fma(ex ? 0.0 : p.y, r, S);
+            }
+            plain(max(q0, whi), q1);
+            part[wave][lane] = S;
+            __syncthreads();
+            if (wave == 0 && i < K) {
+                double sum = 0.0;
+                for (int w = 0; w < W; w++) sum += part[w][lane];
+                rlam[s + i] = sum;
+            }
+            __syncthreads();  // part[] is read before the next tile rewrites it
+        }
+        tbase += nt;
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(kWave * W)
 __attribute__((amdgpu_waves_per_eu(kStreamSlots == 1 ? 4 : 2, kStreamSlots == 1 ? 4 : 2)))
@@ -1490,6 +1596,9 @@ void k_secular_stream(
                     if (i < 0 && base + rank < K) {
                         i = base + rank;
                         st.init(E, K, i, rho);
+                        // the midpoint pass ran in k_secular_mid
+                        SecSums s0 = {rlam[s + i], 0.0, 0.0, 0.0, 0.0};
+                        st.advance(E, s0);
                     }
                 }
                 const bool active = (i >= 0);
