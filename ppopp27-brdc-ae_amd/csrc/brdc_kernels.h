@@ -291,14 +291,33 @@ __global__ __launch_bounds__(LP) void k_leaf(
     bool failed = false;
     const int maxit = 30 * m;
     int iter = 0;
+    // e[k] is negligible when |e[k]| <= eps (|d[k]| + |d[k+1]|)
+    auto negligible = [](double ek, double dk, double dk1) {
+        return fabs(ek) <= kEps * (fabs(dk) + fabs(dk1));
+    };
+    // the first negligible off-diagonal at or after l (m - 1 if none), four
+    // entries a step so that their loads are in flight together
+    auto scan = [&](int from) {
+        int k = from;
+        for (; k + 4 <= m - 1; k += 4) {
+            double dv[5], ev[4];
+#pragma unroll
+            for (int u = 0; u < 5; u++) dv[u] = sd_(k + u);
+#pragma unroll
+            for (int u = 0; u < 4; u++) ev[u] = se_(k + u);
+            int hit = 4;
+#pragma unroll
+            for (int u = 3; u >= 0; u--)
+                if (negligible(ev[u], dv[u], dv[u + 1])) hit = u;
+            if (hit < 4) return k + hit;
+        }
+        for (; k < m - 1; k++)
+            if (negligible(se_(k), sd_(k), sd_(k + 1))) return k;
+        return m - 1;
+    };
     for (int l = 0; l < m && !failed; l++) {
+        int mm = scan(l);
         for (;;) {
-            // the first negligible off-diagonal at or after l
-            int mm = l;
-            for (; mm < m - 1; mm++) {
-                const double dd = fabs(sd_(mm)) + fabs(sd_(mm + 1));
-                if (fabs(se_(mm)) <= kEps * dd) break;
-            }
             if (mm == l) break;  // d[l] is an eigenvalue
             if (++iter > maxit) {
                 failed = true;
@@ -313,6 +332,13 @@ __global__ __launch_bounds__(LP) void k_leaf(
             // d[k+1] and the two rows' entries k+1 ride in registers from one
             // step to the next; each step loads entry k and stores entry k+1
             double dk1 = sd_(mm), x1 = s0_(mm), y1 = s1_(mm);
+            // The sweep rewrites d[l..mm-1] and e[l..mm-1] and leaves e[mm]
+            // zero, so the next iteration's first negligible off-diagonal is
+            // the lowest k in [l, mm] that tests negligible on the stored
+            // values: each step tests the entries it stores (dnx = the d[k+2]
+            // stored by the step before).
+            int nxt = mm;
+            double dnx = 0.0;
             for (int k = mm - 1; k >= l; k--) {
                 const double ek = se_(k), dk = sd_(k);
                 const double x0 = s0_(k), y0 = s1_(k);
@@ -330,10 +356,14 @@ __global__ __launch_bounds__(LP) void k_leaf(
                 const double rr = rcp_nr(r);
                 sn = f * rr;
                 c = gg * rr;
+                const double ek1 = r;  // the e[k+1] just stored
                 gg = dk1 - p;
                 r = (dk - gg) * sn + 2.0 * c * b;
                 p = sn * r;
-                sd_(k + 1) = gg + p;
+                const double dnew = gg + p;
+                sd_(k + 1) = dnew;
+                if (k + 1 < mm && negligible(ek1, dnew, dnx)) nxt = k + 1;
+                dnx = dnew;
                 gg = c * r - b;
                 // the two kept rows of the eigenvector matrix
                 s0_(k + 1) = sn * x0 + c * x1;
@@ -342,20 +372,32 @@ __global__ __launch_bounds__(LP) void k_leaf(
                 y1 = c * y0 - sn * y1;
                 dk1 = dk;
             }
-            if (underflow) continue;
+            if (underflow) {
+                mm = scan(l);
+                continue;
+            }
             s0_(l) = x1;
             s1_(l) = y1;
-            sd_(l) = dk1 - p;
+            const double dl_new = dk1 - p;
+            sd_(l) = dl_new;
             se_(l) = gg;
             se_(mm) = 0.0;
+            mm = negligible(gg, dl_new, dnx) ? l : nxt;
         }
     }
     if (failed) report_item_failure(fail_site(info, moff, n, it), m);
     // ascending order by rank (ties by position), carrying the two rows
     for (int a = 0; a < m; a++) {
         const double x = sd_(a);
-        int rank = 0;
-        for (int b = 0; b < m; b++) {
+        int rank = 0, b = 0;
+        for (; b + 4 <= m; b += 4) {
+            double y[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) y[u] = sd_(b + u);
+#pragma unroll
+            for (int u = 0; u < 4; u++) rank += (y[u] < x) || (y[u] == x && b + u < a);
+        }
+        for (; b < m; b++) {
             const double y = sd_(b);
             rank += (y < x) || (y == x && b < a);
         }
