@@ -248,6 +248,25 @@ __device__ __forceinline__ double lapy2(double x, double y) {
     return w * sqrt(fma(q, q, 1.0));
 }
 
+// Plane rotation of the QL sweep: r = sqrt(f^2 + g^2), c = g / r, s = f / r
+// from one v_rsq_f64 refined by two Newton steps (about ten dependent
+// operations, against some thirty for a scaled hypotenuse and a division).
+// Inside a sweep |f| and |g| are bounded by the scaled matrix's norm, so
+// f^2 + g^2 cannot overflow; when it underflows to zero (|f|, |g| below
+// 1e-154, far under the deflation threshold) r = 0 takes the sweep's
+// underflow exit, as an exact zero does.
+__device__ __forceinline__ void ql_rotation(double f, double g, double& r, double& c,
+                                            double& s) {
+    const double t = fma(f, f, g * g);
+    double y = __builtin_amdgcn_rsq(t);
+#pragma unroll
+    for (int it = 0; it < 2; it++) y = fma(0.5 * y, fma(-(t * y), y, 1.0), y);
+    const bool zero = (t == 0.0);
+    r = zero ? 0.0 : t * y;
+    c = g * y;
+    s = f * y;
+}
+
 // Lanes (leaves) per block for leaves of at most maxm rows: each lane holds
 // four LDS vectors of maxm doubles, and a block stays within 64 KB.
 __host__ __device__ inline int leaf_lanes(int maxm) { return maxm <= 32 ? kWave : kWave / 2; }
@@ -343,7 +362,8 @@ __global__ __launch_bounds__(LP) void k_leaf(
                 const double ek = se_(k), dk = sd_(k);
                 const double x0 = s0_(k), y0 = s1_(k);
                 const double f = sn * ek, b = c * ek;
-                r = lapy2(f, gg);
+                double cn, sv;
+                ql_rotation(f, gg, r, cn, sv);
                 se_(k + 1) = r;
                 if (r == 0.0) {  // recover from underflow
                     sd_(k + 1) = dk1 - p;
@@ -353,9 +373,8 @@ __global__ __launch_bounds__(LP) void k_leaf(
                     underflow = true;
                     break;
                 }
-                const double rr = rcp_nr(r);
-                sn = f * rr;
-                c = gg * rr;
+                sn = sv;
+                c = cn;
                 const double ek1 = r;  // the e[k+1] just stored
                 gg = dk1 - p;
                 r = (dk - gg) * sn + 2.0 * c * b;
