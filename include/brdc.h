@@ -112,10 +112,12 @@ int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void 
  * bracketed by two HIP events on its own stream.  end synchronises those
  * events and writes, for each of the first nclasses classes in the order
  * norm, prep_e, leaf, prepare, deflate, secular (merges below 2048 poles),
- * wprod, rows, finalize, secular_stream (merges of 2048 poles or more)
- * (BRDC_KERNEL_CLASSES of them), the summed milliseconds and launch counts.
+ * wprod, rows, finalize, secular_stream (merges of 2048 poles or more, every
+ * evaluation after the first), secular_mid (their first, midpoint
+ * evaluation) (BRDC_KERNEL_CLASSES of them), the summed milliseconds and
+ * launch counts.
  */
-#define BRDC_KERNEL_CLASSES 10
+#define BRDC_KERNEL_CLASSES 11
 int brdc_profile_begin(void);
 int brdc_profile_end(double *ms_out, int *launches_out, int nclasses);
 

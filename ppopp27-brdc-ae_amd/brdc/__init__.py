@@ -218,7 +218,9 @@ def profile_kernels(solver: Solver, problems, stream=None):
     FP64 flops}, "bytes": {class: bytes}} summed over the problems, where the
     flop counts use each solve's own per-merge K and secular evaluation
     counts: 7 flops per pole per evaluation (secular for merges below
-    STREAM_MIN_K poles, secular_stream for the others), 3 per pole pair
+    STREAM_MIN_K poles; secular_stream for the others' evaluations after the
+    first), 4 per pole for the first, midpoint evaluation of the merges of
+    STREAM_MIN_K poles or more (secular_mid: the sum alone), 3 per pole pair
     (product-form w, non-root merges), 9 per pole pair (boundary rows,
     non-root merges).
     """
@@ -227,7 +229,8 @@ def profile_kernels(solver: Solver, problems, stream=None):
     nc = len(_lib.KERNEL_CLASSES)
     tot_ms = np.zeros(nc)
     tot_n = np.zeros(nc, dtype=np.int64)
-    flops = {"secular": 0.0, "secular_stream": 0.0, "wprod": 0.0, "rows": 0.0}
+    flops = {"secular": 0.0, "secular_stream": 0.0, "secular_mid": 0.0, "wprod": 0.0,
+             "rows": 0.0}
     nbytes = {k: 0.0 for k in _BYTES_PER_ROW}
     st = stream if stream is not None else torch.cuda.current_stream()
     for (w, d, e) in problems:
@@ -243,10 +246,12 @@ def profile_kernels(solver: Solver, problems, stream=None):
         tot_n += cnt
         stats = solver.merge_stats(st)
         k = stats["k"].astype(np.float64)
-        sec = 7.0 * k * stats["evals"]
+        ev = stats["evals"].astype(np.float64)
         big = stats["k"] >= _lib.STREAM_MIN_K
-        flops["secular"] += float(np.sum(sec[~big]))
-        flops["secular_stream"] += float(np.sum(sec[big]))
+        flops["secular"] += float(np.sum(7.0 * k[~big] * ev[~big]))
+        # every root of a streamed merge has its first evaluation in secular_mid
+        flops["secular_stream"] += float(np.sum(7.0 * k[big] * (ev[big] - k[big])))
+        flops["secular_mid"] += float(np.sum(4.0 * k[big] * k[big]))
         nonroot = (stats["level"] > 0) & (stats["k"] > 2)
         flops["wprod"] += float(np.sum(3.0 * k[nonroot] ** 2))
         flops["rows"] += float(np.sum(9.0 * k[nonroot] ** 2))

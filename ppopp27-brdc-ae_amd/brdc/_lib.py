@@ -26,7 +26,7 @@ EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
            "brdc_batch_grad")
 
 KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
-                  "finalize", "secular_stream")
+                  "finalize", "secular_stream", "secular_mid")
 # merges of at least this many poles go to the streamed secular kernel
 # (kStreamMinK in csrc/brdc.hip)
 STREAM_MIN_K = 2048

@@ -173,7 +173,7 @@ T* at(void* ws, size_t off) {
 // MergeLevelTimer, gpu/src/tridiag_solver.cu:158-359): when enabled on the
 // calling host thread, every launch is bracketed by two events on its stream.
 enum KernelClass { kNorm, kPrepE, kLeaf, kPrepare, kDeflate, kSecular, kWprod, kRows, kFinalize,
-                   kSecularStream, kNumClasses };
+                   kSecularStream, kSecularMid, kNumClasses };
 struct Profiler {
     bool on = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
@@ -400,12 +400,16 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
         }
         BRDC_LAUNCH_CHECK();
         if (streamed) {
-            ProfScope ps(kSecularStream, st);
-            hipLaunchKernelGGL(k_secular_mid<kStreamWaves>,
-                               dim3(std::min(kMidMaxGroups, (n + kWave - 1) / kWave)),
-                               dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
-                               (const double*)ew, (const int*)counts, (const double2*)adz, rlam);
+            {
+                ProfScope ps(kSecularMid, st);
+                hipLaunchKernelGGL(k_secular_mid<kStreamWaves>,
+                                   dim3(std::min(kMidMaxGroups, (n + kWave - 1) / kWave)),
+                                   dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
+                                   (const double*)ew, (const int*)counts, (const double2*)adz,
+                                   rlam);
+            }
             BRDC_LAUNCH_CHECK();
+            ProfScope ps(kSecularStream, st);
             const int groups = std::min(stream_groups(), (n + kWave - 1) / kWave);
             hipLaunchKernelGGL(k_secular_stream<kStreamWaves>, dim3(groups),
                                dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
