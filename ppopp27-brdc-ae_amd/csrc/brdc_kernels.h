@@ -889,6 +889,45 @@ __host__ __device__ inline int waves_for(int K, int Wmax) {
 // root's lanes combine their sums in an xor butterfly.  A wave therefore
 // runs only as many passes as its 64/L roots need, and a small merge is
 // spread over many lanes for latency.
+// A double moved between lanes by a DPP pattern within rows of 16 lanes (a
+// VALU operation, no LDS round trip).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sums over aligned groups of L lanes (L a power of two up to 64) of six
+// values; every lane of a group gets its group's totals.  Each step adds a
+// lane's value and its partner's, with the partners of an xor butterfly on
+// lane bits 0, 1, 2, 3 (DPP quad swaps, then the half-row and row mirrors,
+// which pair lane i with a lane of the other quad and of the other half-row)
+// and bits 4, 5 (LDS shuffles), so the sums are those of the butterfly.
+__device__ __forceinline__ void group_sum6(double (&acc)[6], int L) {
+    if (L >= 2) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += dpp_f64<0xB1>(acc[q]);  // quad [1,0,3,2]
+    }
+    if (L >= 4) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += dpp_f64<0x4E>(acc[q]);  // quad [2,3,0,1]
+    }
+    if (L >= 8) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += dpp_f64<0x141>(acc[q]);  // row half mirror
+    }
+    if (L >= 16) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += dpp_f64<0x140>(acc[q]);  // row mirror
+    }
+    for (int o = 16; o < L; o <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+    }
+}
+
 struct PoleView {
     const double2* __restrict__ dz;  // (d_j, z_j^2)
     __device__ __forceinline__ double d(int j) const { return dz[j].x; }
@@ -976,10 +1015,7 @@ __device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, 
         secular_side(dz, c, st.ev_exlo, L, st.ev_org, st.ev_tau, acc[0], acc[1], acc[2]);
         secular_side(dz, j2 + ((c - j2) & (L - 1)), K, L, st.ev_org, st.ev_tau, acc[3], acc[4],
                      acc[5]);
-        for (int o = 1; o < L; o <<= 1) {
-#pragma unroll
-            for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
-        }
+        group_sum6(acc, L);
         SecSums sums;
         sums.S = acc[0] + acc[3];
         sums.A = acc[3] - acc[0];
@@ -1184,10 +1220,7 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
                 secular_side(dz, j2 + ((c - j2) & (L - 1)), K, L, st.ev_org, st.ev_tau, acc[3],
                              acc[4], acc[5]);
             }
-            for (int o = 1; o < L; o <<= 1) {
-#pragma unroll
-                for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
-            }
+            group_sum6(acc, L);
             SecSums sums;
             sums.S = acc[0] + acc[3];
             sums.A = acc[3] - acc[0];
@@ -1719,10 +1752,7 @@ void k_secular_stream(
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
                     double acc[6] = {lo[r].S, lo[r].D, lo[r].J, hi[r].S, hi[r].D, hi[r].J};
-                    for (int o = 1; o < L; o <<= 1) {
-#pragma unroll
-                        for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
-                    }
+                    group_sum6(acc, L);
 #pragma unroll
                     for (int q = 0; q < 6; q++) part[k][r][q][lane] = acc[q];
                 }
