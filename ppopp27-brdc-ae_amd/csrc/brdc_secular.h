@@ -65,6 +65,41 @@ __host__ __device__ __forceinline__ double fdiv(double a, double b) {
 #endif
 }
 
+// A denominator used several times: fdiv(a, d) == Divisor(d).div(a) in
+// every build (the reciprocal formed once on the device, exact division on
+// the host).
+struct Divisor {
+    double v;
+    __host__ __device__ explicit Divisor(double d) {
+#if defined(__HIP_DEVICE_COMPILE__) || defined(BRDC_HOST_DEVICE_ARITH)
+        v = rcp_nr(d);
+#else
+        v = d;
+#endif
+    }
+    __host__ __device__ double div(double a) const {
+#if defined(__HIP_DEVICE_COMPILE__) || defined(BRDC_HOST_DEVICE_ARITH)
+        return a * v;
+#else
+        return a / v;
+#endif
+    }
+};
+
+// The root-of-a-quadratic step shared by the iterations:
+//   c == 0:       b / a
+//   else a <= 0:  (a - sq) / (2c)
+//   else:         2b / (a + sq),     sq = sqrt(|a^2 - 4bc|)
+// with one square root and one division whichever case applies (lanes of a
+// wave in different cases then share them).
+__host__ __device__ __forceinline__ double quad_step(double a, double b, double c) {
+    const double sq = sqrt(fabs(a * a - 4.0 * b * c));
+    const bool cz = (c == 0.0), an = (a <= 0.0);
+    const double num = cz ? b : (an ? a - sq : 2.0 * b);
+    const double den = cz ? a : (an ? 2.0 * c : a + sq);
+    return fdiv(num, den);
+}
+
 // Two-pole closed form (the DLAED5 role).  vec gets the normalised eigenvector.
 __host__ __device__ inline void secular_two(int i, const double* d, const double* z, double rho,
                                    double* lam, double* v0, double* v1, double* tau_out) {
@@ -199,12 +234,14 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
             zs[k] = zz[k];
         }
     }
+    // the poles are fixed through the iteration: one reciprocal each
+    const Divisor rds[3] = {Divisor(ds[0]), Divisor(ds[1]), Divisor(ds[2])};
     fc = df = ddf = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const double r = rcp_nr(ds[k] - tau);
         const double t1 = zs[k] * r, t2 = t1 * r, t3 = t2 * r;
-        fc += fdiv(t1, ds[k]);
+        fc += rds[k].div(t1);
         df += t2;
         ddf += t3;
     }
@@ -231,12 +268,7 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
         a *= temp;
         b *= temp;
         c *= temp;
-        if (c == 0.0)
-            eta = fdiv(b, a);
-        else if (a <= 0.0)
-            eta = fdiv(a - sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
-        else
-            eta = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
+        eta = quad_step(a, b, c);
         if (f * eta >= 0.0)
             eta = -fdiv(f, df);
         tau += eta;
@@ -253,7 +285,7 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const double r = rcp_nr(ds[k] - tau);
-            const double u1 = zs[k] * r, u2 = u1 * r, u3 = u2 * r, u4 = fdiv(u1, ds[k]);
+            const double u1 = zs[k] * r, u2 = u1 * r, u3 = u2 * r, u4 = rds[k].div(u1);
             fc += u4;
             erretm += fabs(u4);
             df += u2;
@@ -392,8 +424,9 @@ struct SecularSolve {
             dphi = s.dphi;
             const double err = s.J - (double)ii * s.S;
             const double dii = -tau;
-            const double temp = fdiv(zii2, dii);
-            dw = dpsi + dphi + fdiv(temp, dii);
+            const Divisor rdii(dii);
+            const double temp = rdii.div(zii2);
+            dw = dpsi + dphi + rdii.div(temp);
             w0 = rhoinv + S;
             w = w0 + temp;
             erretm = 8.0 * A + err + 2.0 * rhoinv + 3.0 * fabs(temp) + (k >= 2 ? fabs(tau) : 0.0) * dw;
@@ -453,23 +486,17 @@ struct SecularSolve {
                 }
                 double a = (dl_i + dl_ip1) * w - dl_i * dl_ip1 * dw;
                 const double b = dl_i * dl_ip1 * w;
-                if (c == 0.0) {
-                    if (a == 0.0) {
-                        if (niter == 2 || !swtch) {
-                            if (orgati)
-                                a = zi2 + dl_ip1 * dl_ip1 * (dpsi + dphi);
-                            else
-                                a = zip12 + dl_i * dl_i * (dpsi + dphi);
-                        } else {
-                            a = dl_i * dl_i * dpsi + dl_ip1 * dl_ip1 * dphi;
-                        }
+                if (c == 0.0 && a == 0.0) {
+                    if (niter == 2 || !swtch) {
+                        if (orgati)
+                            a = zi2 + dl_ip1 * dl_ip1 * (dpsi + dphi);
+                        else
+                            a = zip12 + dl_i * dl_i * (dpsi + dphi);
+                    } else {
+                        a = dl_i * dl_i * dpsi + dl_ip1 * dl_ip1 * dphi;
                     }
-                    eta = fdiv(b, a);
-                } else if (a <= 0.0) {
-                    eta = fdiv(a - sqrt(fabs(a * a - 4.0 * b * c)), 2.0 * c);
-                } else {
-                    eta = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
                 }
+                eta = quad_step(a, b, c);
             } else {
                 const double dl_m = delta_at(dm1), dl_p = delta_at(dp1);
                 const double temp = rhoinv + S;
