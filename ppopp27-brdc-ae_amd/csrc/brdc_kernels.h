@@ -1141,9 +1141,15 @@ __global__ __launch_bounds__(kScanThreads) void k_root_scan(int merges, int kstr
 
 // All roots of the level's merges with K < kstream, packed (see k_root_scan):
 // L lanes per root from the level's mean K, each wave a contiguous run of
-// root groups.  A lane's merge is found by a walk over roff from the run's
-// first merge (found once by binary search).  Poles are read from global
-// memory (the L1 cache holds the few merges a group spans).
+// roots.  The wave's 64/L root positions take the run's roots in order, and
+// a position whose root has converged takes the next root of the run at the
+// start of the next pass, so the wave's passes follow the roots' mean
+// evaluation count rather than every group's slowest root.  A position's
+// merge is found by a walk over roff from the run's first merge (found once
+// by binary search); roots of merges of at most two poles take the closed
+// form as they are claimed.  Poles are read from global memory (the L1 cache
+// holds the few merges a wave spans).  Each root's arithmetic is that of a
+// lone root at L lanes, whatever it runs beside.
 template <int WL>
 __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_secular_packed(
     Geo g, int merges, const int* __restrict__ roff, const double* __restrict__ ew,
@@ -1154,91 +1160,103 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
     const int total = uniform(roff[merges]);
     if (total == 0) return;
     const int L = secular_lanes((total + merges - 1) / merges, merges);
-    const int rpg = kWave / L;  // roots per group
+    const int rpg = kWave / L;  // root positions per wave
     const long long groups = (total + rpg - 1) / rpg;
     const int nwave = gridDim.x * WL;
     const int wave = uniform(blockIdx.x * WL + (int)(threadIdx.x / kWave));
     const int lane = threadIdx.x & (kWave - 1);
-    const int c = lane & (L - 1);
+    const int c = lane & (L - 1), lead = lane - c;  // lane c of the position at lanes lead..
     const int g0 = (int)(groups * wave / nwave), g1 = (int)(groups * (wave + 1) / nwave);
     if (g0 >= g1) return;
-    // the merge holding root g0 * rpg: the last m with roff[m] <= R
-    int mrun;
+    // this wave's run of roots [R0, R1)
+    const int R0 = g0 * rpg, R1 = (int)min<long long>((long long)g1 * rpg, total);
+    // the merge holding root R0: the last m with roff[m] <= R0
+    int m;
     {
-        const int R = g0 * rpg;
-        int lo = 0, hi = merges;  // roff[lo] <= R < roff[hi]
+        int lo = 0, hi = merges;  // roff[lo] <= R0 < roff[hi]
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (roff[mid] <= R) lo = mid;
+            if (roff[mid] <= R0) lo = mid;
             else hi = mid;
         }
-        mrun = uniform(lo);
+        m = uniform(lo);
     }
-    for (int grp = g0; grp < g1; grp++) {
-        const int R0 = grp * rpg;
-        while (roff[mrun + 1] <= R0) mrun++;  // wave-uniform walk
-        const int R = R0 + lane / L;
-        const bool has = R < total;
-        int m = mrun;
-        if (has)
-            while (roff[m + 1] <= R) m++;
-        const int K = has ? counts[kCountStride * m] : 0;
-        const int i = R - roff[m];
-        const Item it = g.item(m);
-        const int s = it.s, sz = it.sz;
-        const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
-        int iters = 0;
-        bool active = has && K >= 3;
-        if (has && K <= 2 && c == 0) {  // two-pole closed form / single pole
-            if (K == 1) {
-                const double z0 = az[s];
-                rlam[s] = ad[s] + rho * z0 * z0;
-                rs[s] = make_double2(ad[s], rho * z0 * z0);
-                rn1[s] = 1.0;
-                rn2[s] = 0.0;
-            } else {
-                double lam, v0, v1, tau;
-                secular_two(i, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
-                rlam[s + i] = lam;
-                rs[s + i] = make_double2(ad[s], tau);
-                rn1[s + i] = v0;
-                rn2[s + i] = v1;
+    int next = R0;  // the run's next unclaimed root (wave-uniform)
+    // this position's root: its merge m (walks forward), K, index i, rows s
+    int K = 0, i = 0, s = 0, sz = 0;
+    double rho = 0.0;
+    bool active = false;
+    const double2* __restrict__ dz = adz;
+    SecularSolve<PoleView> st;
+    for (;;) {
+        const unsigned long long need = __ballot(!active && c == 0);
+        if (need && next < R1) {
+            // one root per idle position, in rank order
+            const int rank = (int)__popcll(need & ((1ull << lead) - 1ull));
+            const int R = next + rank;
+            next = min(next + (int)__popcll(need), R1);
+            if (!active && R < R1) {
+                while (roff[m + 1] <= R) m++;
+                K = counts[kCountStride * m];
+                i = R - roff[m];
+                const Item it = g.item(m);
+                s = it.s;
+                sz = it.sz;
+                rho = fabs(2.0 * ew[s + sz / 2 - 1]);
+                dz = adz + s;
+                if (K >= 3) {
+                    st.init(PoleView{dz}, K, i, rho);
+                    active = true;
+                } else if (c == 0) {  // two-pole closed form / single pole
+                    if (K == 1) {
+                        const double z0 = az[s];
+                        rlam[s] = ad[s] + rho * z0 * z0;
+                        rs[s] = make_double2(ad[s], rho * z0 * z0);
+                        rn1[s] = 1.0;
+                        rn2[s] = 0.0;
+                    } else {
+                        double lam, v0, v1, tau;
+                        secular_two(i, ad + s, az + s, rho, &lam, &v0, &v1, &tau);
+                        rlam[s + i] = lam;
+                        rs[s + i] = make_double2(ad[s], tau);
+                        rn1[s + i] = v0;
+                        rn2[s + i] = v1;
+                    }
+                    if (i == 0) atomicAdd(&counts_rw[kCountStride * m + 3], 1);
+                }
             }
-            iters = (i == 0) ? 1 : 0;
         }
-        const double2* __restrict__ dz = adz + s;
+        if (!__any(active)) {
+            if (next >= R1) break;  // the run is done
+            continue;               // the positions took closed-form roots only
+        }
         const PoleView E{dz};
-        SecularSolve<PoleView> st;
-        st.init(E, active ? K : 3, active ? i : 0, rho);
-        for (;;) {
-            if (!__any(active)) break;
-            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            if (active) {
-                const int j2 = st.ev_exlo + st.ev_nex;
-                secular_side(dz, c, st.ev_exlo, L, st.ev_org, st.ev_tau, acc[0], acc[1],
-                             acc[2]);
-                secular_side(dz, j2 + ((c - j2) & (L - 1)), K, L, st.ev_org, st.ev_tau, acc[3],
-                             acc[4], acc[5]);
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (active) {
+            const int j2 = st.ev_exlo + st.ev_nex;
+            secular_side(dz, c, st.ev_exlo, L, st.ev_org, st.ev_tau, acc[0], acc[1], acc[2]);
+            secular_side(dz, j2 + ((c - j2) & (L - 1)), K, L, st.ev_org, st.ev_tau, acc[3],
+                         acc[4], acc[5]);
+        }
+        group_sum6(acc, L);
+        SecSums sums;
+        sums.S = acc[0] + acc[3];
+        sums.A = acc[3] - acc[0];
+        sums.dpsi = acc[1];
+        sums.dphi = acc[4];
+        sums.J = acc[2] + acc[5];
+        if (active && st.advance(E, sums)) {
+            active = false;
+            if (c == 0) {
+                const RootState& r = st.out;
+                rlam[s + i] = r.lambda;
+                rs[s + i] = make_double2(dz[r.org].x, r.tau);
+                rn1[s + i] = r.near1;
+                rn2[s + i] = r.near2;
+                if (r.iters) atomicAdd(&counts_rw[kCountStride * m + 3], r.iters);
+                if (r.info) report_item_failure(fail_site(info, moff, g.n, g.item(m)), sz);
             }
-            group_sum6(acc, L);
-            SecSums sums;
-            sums.S = acc[0] + acc[3];
-            sums.A = acc[3] - acc[0];
-            sums.dpsi = acc[1];
-            sums.dphi = acc[4];
-            sums.J = acc[2] + acc[5];
-            if (active && st.advance(E, sums)) active = false;
         }
-        if (has && K >= 3 && c == 0) {
-            const RootState& r = st.out;
-            rlam[s + i] = r.lambda;
-            rs[s + i] = make_double2(dz[r.org].x, r.tau);
-            rn1[s + i] = r.near1;
-            rn2[s + i] = r.near2;
-            iters = r.iters;
-            if (r.info) report_item_failure(fail_site(info, moff, g.n, it), sz);
-        }
-        if (iters) atomicAdd(&counts_rw[kCountStride * m + 3], iters);
     }
 }
 
