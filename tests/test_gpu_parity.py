@@ -212,6 +212,23 @@ def test_streamed_merges_are_reproducible(gpu, orc):
             np.testing.assert_array_equal(out.cpu().numpy(), w0)
 
 
+@pytest.mark.parametrize("n", [4098, 4226, 6146, 8194])
+def test_streamed_merge_edge_sizes(gpu, n):
+    """Toeplitz D=2, E=1 keeps K = n/2 at its top merge, so these orders put
+    K just above the streaming threshold (2049), one root past a 64-root tile
+    (2113 = 33*64 + 1: a tile holding the last root alone, whose excluded
+    poles K-2, K-1 start below the tile), one pole past a 128-pole chunk
+    (3073) and 4097.  Closed form and repeat-exactness."""
+    d, e = np.full(n, 2.0), np.full(n - 1, 1.0)
+    ref = np.sort(2.0 + 2.0 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
+    w0, info, s = solve(gpu, d, e)
+    assert info == 0 and rel_err(w0, ref) <= TOL
+    st = s.merge_stats()
+    assert int(st["k"].max()) == n // 2 and n // 2 >= 2048
+    w1, _, _ = solve(gpu, d, e)
+    np.testing.assert_array_equal(w1, w0)
+
+
 def test_matches_rocsolver_on_the_same_gpu(gpu):
     """rocSOLVER's STERF and STEDC (values only) as an independent GPU
     reference at a size they finish quickly."""
