@@ -63,27 +63,44 @@ __device__ __forceinline__ double rcp_chain(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
-// Sum over the wave's 64 lanes of each of NV values (NV a power of two up to
-// 64), scattered: returns in lane L the total of value L / (64 / NV) (the
-// 64 / NV lanes that share a value agree).  Halving on lane bits 5, 4, ...
-// costs NV/2 + NV/4 + ... + 1 exchanges, plus log2(64 / NV) for the lanes
-// that share a value.
-template <int NV>
-__device__ __forceinline__ double reduce_scatter(double* v, int lane) {
+// Sum over the wave's 64 lanes of each of 32 values v[0..31], scattered:
+// returns in lane L the total of value L / 2 (both lanes of a pair agree).
+// Each halving keeps, in every lane, the half of its values that one lane bit
+// selects and adds the partner lane's copy of that half.  The partners differ
+// in bit 5 (half swap), bit 4 (row swap), then bit 3, 2, 1 and lower bits
+// (row mirror, half-row mirror, quad swap: lane i pairs with i ^ 15, i ^ 7,
+// i ^ 2), and a final quad swap adds the pair i, i ^ 1; the pairings span all
+// 64 lanes, so every total is over the whole wave, and no value leaves the
+// VALU (the first two halvings need no select: the swap hands every lane its
+// own half and the partner's copy of it).
+__device__ __forceinline__ double reduce_scatter32(double (&v)[32], int lane) {
 #pragma unroll
-    for (int h = NV / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
-        const bool hi = (lane & bit) != 0;
-#pragma unroll
-        for (int i = 0; i < h; i++) {
-            const double keep = hi ? v[h + i] : v[i];
-            const double send = hi ? v[i] : v[h + i];
-            v[i] = keep + __shfl_xor(send, bit, kWave);
-        }
+    for (int i = 0; i < 16; i++) {  // bit 5: value i (bit clear) or 16 + i (set)
+        lane_swap_f64<true>(v[i], v[16 + i]);
+        v[i] = v[i] + v[16 + i];
     }
-    double t = v[0];
 #pragma unroll
-    for (int bit = kWave / NV / 2; bit >= 1; bit >>= 1) t += __shfl_xor(t, bit, kWave);
-    return t;
+    for (int i = 0; i < 8; i++) {  // bit 4: value ... + i or ... + 8 + i
+        lane_swap_f64<false>(v[i], v[8 + i]);
+        v[i] = v[i] + v[8 + i];
+    }
+    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0, b1 = (lane & 2) != 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {  // bit 3, partner i ^ 15
+        const double keep = b3 ? v[4 + i] : v[i];
+        const double send = b3 ? v[i] : v[4 + i];
+        v[i] = keep + dpp_f64<0x140>(send);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {  // bit 2, partner i ^ 7
+        const double keep = b2 ? v[2 + i] : v[i];
+        const double send = b2 ? v[i] : v[2 + i];
+        v[i] = keep + dpp_f64<0x141>(send);
+    }
+    const double keep = b1 ? v[1] : v[0];  // bit 1, partner i ^ 2
+    const double send = b1 ? v[0] : v[1];
+    const double t = keep + dpp_f64<0x4E>(send);
+    return t + dpp_f64<0xB1>(t);  // the pair i, i ^ 1
 }
 
 // Work items of the eigenvector kernel (k_grad_vectors).  A task is 64
@@ -428,11 +445,11 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 v[j] = sc * M[j] * M[j];
                 v[kSeg + j] = 2.0 * sc * M[j] * znext;
             }
-            // reduce-scatter over the wave: at each halving the lane keeps the
-            // half of its values its lane bit selects and adds the partner's
-            // copy of it; afterwards lane L holds value L / kShare
+            // reduce-scatter over the wave: afterwards lane L holds the total
+            // of value L / kShare
+            static_assert(2 * kSeg == 32, "the reduce-scatter takes 32 values");
             constexpr int kShare = kWave / (2 * kSeg);  // lanes holding each total
-            const double total = reduce_scatter<2 * kSeg>(v, lane);
+            const double total = reduce_scatter32(v, lane);
             if ((lane & (kShare - 1)) == 0) {
                 const int idx = lane / kShare;
                 const int row = idx & (kSeg - 1);

@@ -899,12 +899,35 @@ __device__ __forceinline__ double dpp_f64(double x) {
     return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
 
+// gfx950's lane swaps of two registers a, b (VALU, no LDS round trip):
+//   half swap (v_permlane32_swap): lanes 32..63 of a <-> lanes 0..31 of b,
+//   row swap (v_permlane16_swap): rows 1 and 3 of a <-> rows 0 and 2 of b
+// (rows of 16 lanes).  Afterwards a + b in every lane is the lane's value of
+// a (b) plus its partner's across lane bit 5 (4), where a lane whose bit is
+// clear keeps a's values and a lane whose bit is set keeps b's: one step of
+// a reduce-scatter; with a == b it is one step of the xor butterfly.  Every
+// lane of the wave must be active.
+template <bool kHalf>
+__device__ __forceinline__ void lane_swap_f64(double& a, double& b) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const unsigned xl = (unsigned)x, xh = (unsigned)(x >> 32);
+    const unsigned yl = (unsigned)y, yh = (unsigned)(y >> 32);
+    auto swap = [](unsigned p, unsigned q) {
+        if constexpr (kHalf) return __builtin_amdgcn_permlane32_swap(p, q, false, false);
+        else return __builtin_amdgcn_permlane16_swap(p, q, false, false);
+    };
+    const auto lo = swap(xl, yl);
+    const auto hi = swap(xh, yh);
+    a = __longlong_as_double((long long)(unsigned)lo[0] | ((long long)hi[0] << 32));
+    b = __longlong_as_double((long long)(unsigned)lo[1] | ((long long)hi[1] << 32));
+}
+
 // Sums over aligned groups of L lanes (L a power of two up to 64) of six
 // values; every lane of a group gets its group's totals.  Each step adds a
 // lane's value and its partner's, with the partners of an xor butterfly on
 // lane bits 0, 1, 2, 3 (DPP quad swaps, then the half-row and row mirrors,
 // which pair lane i with a lane of the other quad and of the other half-row)
-// and bits 4, 5 (LDS shuffles), so the sums are those of the butterfly.
+// and bits 4, 5 (row and half swaps), so the sums are those of the butterfly.
 __device__ __forceinline__ void group_sum6(double (&acc)[6], int L) {
     if (L >= 2) {
 #pragma unroll
@@ -922,9 +945,21 @@ __device__ __forceinline__ void group_sum6(double (&acc)[6], int L) {
 #pragma unroll
         for (int q = 0; q < 6; q++) acc[q] += dpp_f64<0x140>(acc[q]);  // row mirror
     }
-    for (int o = 16; o < L; o <<= 1) {
+    if (L >= 32) {
 #pragma unroll
-        for (int q = 0; q < 6; q++) acc[q] += __shfl_xor(acc[q], o, kWave);
+        for (int q = 0; q < 6; q++) {
+            double a = acc[q], b = acc[q];
+            lane_swap_f64<false>(a, b);
+            acc[q] = a + b;
+        }
+    }
+    if (L >= 64) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            double a = acc[q], b = acc[q];
+            lane_swap_f64<true>(a, b);
+            acc[q] = a + b;
+        }
     }
 }
 
