@@ -485,18 +485,15 @@ struct brdc_batch_plan {
 namespace brdc {
 namespace {
 
-// Gradient scratch: per task three checkpoint arrays of (segments + 1) x 64
-// doubles; per task 64 twist indices and two |z|^2 halves; per gamma item 64
+// Gradient scratch: per task the checkpoint arrays of (segments + 1) x 64
+// doubles (grad_cp_slot); per task 64 twist indices and two |z|^2 halves; per gamma item 64
 // best |gamma| values and rows; then the per-task partial sums.
 struct GradShape {
     size_t cp, rarr, ndarr, nuarr, garr, rgarr, partial, total;
 };
 
 // checkpoint doubles of one task of an n-row matrix
-inline long long grad_slot(int n) {
-    const long long segs = (std::max(n, 1) + kSeg - 1) / kSeg;
-    return 3 * (segs + 1) * kWave;
-}
+inline long long grad_slot(int n) { return grad_cp_slot(n); }
 
 GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles, int ngamma) {
     GradShape G{};

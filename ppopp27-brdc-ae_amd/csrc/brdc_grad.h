@@ -37,6 +37,17 @@ __host__ __device__ inline int grad_parts(int n) {
     return p < 1 ? 1 : (p > 64 ? 64 : p);
 }
 
+// Checkpoint doubles of one task of an n-row matrix: (segments + 1) x 64 per
+// array; 3 arrays (D+, D-, z at segment starts), and for a long matrix 5 more
+// (the z scan's segment transfers: down PD, QD; up U0, AU, QU).
+__host__ __device__ inline long long grad_cp_slot(int n) {
+    const long long segs = ((n > 0 ? n : 1) + kSeg - 1) / kSeg;
+    return (grad_parts(n) > 1 ? 8 : 3) * (segs + 1) * 64;
+}
+// partial products of z above this bound mark a segment's transfer as unusable
+// (their squares and sums stay finite below it)
+constexpr double kScanBig = 0x1p400;
+
 // Task t of a batch: matrix b with tprefix[b] <= t < tprefix[b+1], first
 // eigenvalue 64 * (t - tprefix[b]).  One matrix: tprefix == nullptr.
 __device__ __forceinline__ int grad_task_matrix(const int* tprefix, int nmat, int t) {
@@ -54,10 +65,12 @@ __device__ __forceinline__ double pivot_guard(double x, double pivmin) {
     return (fabs(x) < pivmin) ? -pivmin : x;
 }
 
-// Reciprocal on the factorizations' dependent chains: v_rcp_f64 and one
-// Newton step (relative error below 2^-50).  Each step of the stationary and
-// progressive factorizations is then exact for data perturbed by a few ulp,
-// which moves the recomputed eigenvector by O(n eps / gap) as rounding does.
+// Reciprocal of a pivot for the eigenvector's ratios: v_rcp_f64 and one
+// Newton step (relative error below 2^-50; the pivot chains use the same
+// refinement in the factor form r (2 - x r)).  Each step of the stationary
+// and progressive factorizations is then exact for data perturbed by a few
+// ulp, which moves the recomputed eigenvector by O(n eps / gap) as rounding
+// does.
 __device__ __forceinline__ double rcp_chain(double x) {
     const double r = __builtin_amdgcn_rcp(x);
     return fma(r, fma(-x, r, 1.0), r);
@@ -131,9 +144,10 @@ enum GradKind { kGAll, kGChainF, kGChainB, kGGamma, kGZDown, kGZUp, kGContrib };
 //       [sa, sb) (kGGamma, kGContrib) or the task's first gamma item and its
 //       count (kGZDown, kGZUp); nullptr (one matrix of n_one rows): item q of
 //       launch lmode is derived from the task count and parts = grad_parts.
-//   cp: per task 3 arrays of (segments + 1) x 64 doubles at cpoff[b] + (t -
-//       tprefix[b]) * slot_b (cpoff == nullptr: t * slot_b): cpF[s] = D+ and
-//       cpB[s] = D- at row s*kSeg, zc[s] = z at row s*kSeg.
+//   cp: per task the arrays of (segments + 1) x 64 doubles at cpoff[b] + (t -
+//       tprefix[b]) * slot_b (cpoff == nullptr: t * slot_b), slot_b =
+//       grad_cp_slot(n_b): cpF[s] = D+ and cpB[s] = D- at row s*kSeg, zc[s] =
+//       z at row s*kSeg; a long matrix adds its segments' z transfers.
 //   rarr, ndarr, nuarr: per task 64 twists and the two halves of |z|^2;
 //   garr, rgarr: per gamma item 64 best |gamma| values and their rows.
 //   partial: per task 2 * n_b doubles at poff[b] + (t - tprefix[b]) * 2 * n_b.
@@ -207,25 +221,33 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             continue;
         }
         const int S = (n + kSeg - 1) / kSeg;  // segments; only the last may be short
-        const long long slot = 3LL * (S + 1) * kWave;
+        const long long slot = grad_cp_slot(n), arr = (long long)(S + 1) * kWave;
         double* __restrict__ cpF = cpbuf + (cpoff ? cpoff[b] : 0) + (long long)(t - t0) * slot;
-        double* __restrict__ cpB = cpF + (long long)(S + 1) * kWave;
-        double* __restrict__ zc = cpB + (long long)(S + 1) * kWave;
+        double* __restrict__ cpB = cpF + arr;
+        double* __restrict__ zc = cpB + arr;
+        // segment transfers (long matrices): written by the twist search
+        double* __restrict__ tPD = zc + arr;
+        double* __restrict__ tQD = tPD + arr;
+        double* __restrict__ tU0 = tQD + arr;
+        double* __restrict__ tAU = tU0 + arr;
+        double* __restrict__ tQU = tAU + arr;
         const long long tl = (long long)t * kWave + lane;
         auto at = [&](int s) { return (long long)s * kWave + lane; };
         // D and E reach the segments through a window of kWin rows in this
         // wave's LDS (refilled with coalesced loads when a segment leaves it;
         // one wave's LDS operations complete in order, so no barrier).  A
-        // segment reads D of rows lo..lo+16 and E of rows lo-1..lo+15, clamped
-        // to the matrix (rows outside it only feed values that are never used).
+        // segment reads D of rows lo..lo+16 and E of rows lo-1..lo+15: D
+        // clamped to the matrix, E zero outside rows 0..n-2 (rows outside the
+        // matrix only feed values that are never used, except that the zero E
+        // of row n-1 starts the top segment's D- chain).
         int wbase = -(1 << 30);
         auto window = [&](int lo, bool up) {
             if (lo - 1 >= wbase && lo + kSeg + 1 <= wbase + kWin) return;
             wbase = up ? lo - 1 : lo + kSeg + 1 - kWin;
             for (int m = lane; m < kWin; m += kWave) {
-                const int row = min(max(wbase + m, 0), n - 1);
-                sD[m] = d[off + row];
-                sE[m] = e[off + min(row, n - 2)];
+                const int row = wbase + m;
+                sD[m] = d[off + min(max(row, 0), n - 1)];
+                sE[m] = (row >= 0 && row <= n - 2) ? e[off + row] : 0.0;
             }
         };
         double dv[kSeg + 1], ev[kSeg], eprev;
@@ -238,33 +260,54 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             for (int j = 0; j < kSeg; j++) ev[j] = sE[o + j];
             eprev = sE[o - 1];
         };
-        auto step = [&](double ek, double x, double dk) {  // dk - ek^2 / x, guarded
-            return pivot_guard(fma(-ek * ek, rcp_chain(x), dk - mu), pivmin);
+        // One step of a pivot chain: dk - mu - ek^2 / guard(x).  The chain
+        // carries unguarded values and the guard acts as a select of the step
+        // from -pivmin (rg below), decided from x while its reciprocal is in
+        // flight; 1/x is v_rcp_f64 refined by one Newton factor 2 - x r.
+        const double rpv = __builtin_amdgcn_rcp(-pivmin);
+        const double wpv = fma(pivmin, rpv, 2.0);
+        auto step = [&](double ek, double x, double dk) {
+            const double b = ek * ek, c = dk - mu;
+            const double r = __builtin_amdgcn_rcp(x);
+            const double w = fma(-x, r, 2.0);
+            const double y = fma(-(b * r), w, c);
+            const double rg = fma(-(b * rpv), wpv, c);
+            return (fabs(x) < pivmin) ? rg : y;
         };
-        // P[j] = D+ of row lo+j from D+_lo
+        auto guard = [&](double x) { return pivot_guard(x, pivmin); };
+        // P[j] = D+ of row lo+j from D+_lo; returns D+ of row lo+16 (unguarded)
         double P[kSeg], M[kSeg];
         auto seg_dp = [&](double x) {
 #pragma unroll
             for (int j = 0; j < kSeg; j++) {
-                P[j] = x;
+                P[j] = guard(x);
                 x = step(ev[j], x, dv[j + 1]);
             }
+            return x;
         };
         // M[j] = D- of row lo+j: the top segment starts at D_{n-1} - mu, the
         // others from D- of row lo+16 (xhi)
         auto seg_dm = [&](int len, bool top, double xhi) {
-            double x = xhi;
+            // the top segment: E is 0 from row n-1 on, so the step into row
+            // n-1 from any finite value gives D_{n-1} - mu exactly
+            double x = top ? 1.0 : xhi;
 #pragma unroll
             for (int j = kSeg - 1; j >= 0; j--) {
-                const double rec = step(ev[j], x, dv[j]);
-                const double start = top ? pivot_guard(dv[j] - mu, pivmin) : rec;
-                x = (j > len - 1) ? x : ((j == len - 1) ? start : rec);
-                M[j] = x;
+                x = step(ev[j], x, dv[j]);
+                M[j] = guard(x);
             }
         };
         // gamma over segments [sa, sb), bottom up, rows high to low: the first
-        // strictly smaller |gamma| wins (the sequential scan's tie rule)
-        auto gamma_scan = [&](int sa, int sb, double& gbest, int& rbest, bool store_b) {
+        // strictly smaller |gamma| wins (the sequential scan's tie rule).
+        // store_t: also each segment's z transfers for the z scan.  Down (rows
+        // below the twist): z_lo = PD z_{lo+16} with the factors f_k = -E_k /
+        // D+_k, and QD = the segment's sum of (z_k / z_{lo+16})^2.  Up (rows
+        // above it): z_lo = U0 z_{lo-1} and z_{lo+15} = AU z_{lo-1} with the
+        // factors g_k = -E_{k-1} / D-_k, and QU = the sum of (z_k / z_{lo-1})^2.
+        // A partial product above kScanBig (or not finite) stores Q = -1, and
+        // the scan walks the rows instead.
+        auto gamma_scan = [&](int sa, int sb, double& gbest, int& rbest, bool store_b,
+                              bool store_t) {
             double xhi = (sb < S) ? cpB[at(sb)] : 0.0;
             double cpn = cpF[at(sb - 1)];
             for (int s = sb - 1; s >= sa; s--) {
@@ -283,18 +326,41 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 }
                 if (store_b) cpB[at(s)] = M[0];
                 xhi = M[0];
+                if (store_t) {
+                    double pd = 1.0, qd = 0.0;
+                    bool okd = true;
+#pragma unroll
+                    for (int j = kSeg - 1; j >= 0; j--) {
+                        pd *= -(ev[j] * rcp_chain(P[j]));
+                        qd = fma(pd, pd, qd);
+                        okd = okd && fabs(pd) <= kScanBig;
+                    }
+                    double au = 1.0, qu = 0.0, u0 = 1.0;
+                    bool oku = true;
+#pragma unroll
+                    for (int j = 0; j < kSeg; j++) {
+                        au *= -(((j == 0) ? eprev : ev[j - 1]) * rcp_chain(M[j]));
+                        if (j == 0) u0 = au;
+                        const bool in = j < len;
+                        qu = in ? fma(au, au, qu) : qu;
+                        oku = oku && (!in || fabs(au) <= kScanBig);
+                    }
+                    tPD[at(s)] = pd;
+                    tQD[at(s)] = okd ? qd : -1.0;
+                    tU0[at(s)] = u0;
+                    tAU[at(s)] = au;
+                    tQU[at(s)] = oku ? qu : -1.0;
+                }
             }
         };
         // pass F
         if (kind == kGAll || kind == kGChainF) {
-            double x = pivot_guard(D[0] - mu, pivmin);
+            double x = D[0] - mu;
             for (int s = 0; s < S; s++) {
                 const int lo = s * kSeg;
                 load_seg(lo, true);
-                cpF[at(s)] = x;
-                seg_dp(x);
-                // D+ of row lo+16 continues the chain (P holds rows lo..lo+15)
-                x = step(ev[kSeg - 1], P[kSeg - 1], dv[kSeg]);
+                cpF[at(s)] = guard(x);
+                x = seg_dp(x);  // D+ of row lo+16 continues the chain
             }
             if (kind == kGChainF) continue;
         }
@@ -313,12 +379,12 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         int r = n - 1;
         if (kind == kGAll) {  // pass B with the twist search
             double gbest = HUGE_VAL;
-            gamma_scan(0, S, gbest, r, true);
+            gamma_scan(0, S, gbest, r, true, false);
         }
         if (kind == kGGamma) {  // the twist search over [ia, ib)
             double gbest = HUGE_VAL;
             int rb = n - 1;
-            if (ia < ib) gamma_scan(ia, ib, gbest, rb, false);
+            if (ia < ib) gamma_scan(ia, ib, gbest, rb, false, true);
             garr[(long long)q * kWave + lane] = gbest;
             rgarr[(long long)q * kWave + lane] = rb;
             continue;
@@ -333,16 +399,22 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 r = better ? rgarr[x] : r;
             }
         }
-        // pass Z: z with z_r = 1, |z|^2, and zc[s] = z at each segment start
-        double nrm = 1.0;
-        if (kind == kGAll || kind == kGZDown) {
-            // rows below r: z_k = -(E_k / D+_k) z_{k+1}, segments from the top down
-            double z = 1.0;
-            double cpn = cpF[at(S - 1)];
-            for (int s = S - 1; s >= 0; s--) {
+        // pass Z: z with z_r = 1, |z|^2, and zc[s] = z at each segment start.
+        // Segments wholly on the other side of every lane's twist hold z = 1
+        // and add nothing, so the walks start at the lanes' extreme twist
+        // segments.
+        const int sr = r / kSeg;  // this lane's twist segment
+        auto wave_max = [&](int v) {
+            for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+            return uniform(v);
+        };
+        // rows below r in segments s_hi down to s_lo: z_k = -(E_k / D+_k) z_{k+1}
+        auto walk_down = [&](int s_hi, int s_lo, double& z, double& nrm) {
+            double cpn = cpF[at(s_hi)];
+            for (int s = s_hi; s >= s_lo; s--) {
                 const int lo = s * kSeg, len = min(kSeg, n - lo);
                 const double cpc = cpn;
-                if (s > 0) cpn = cpF[at(s - 1)];
+                if (s > s_lo) cpn = cpF[at(s - 1)];
                 load_seg(lo, false);
                 seg_dp(cpc);
 #pragma unroll
@@ -354,18 +426,11 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 }
                 if (lo <= r) zc[at(s)] = z;
             }
-            if (kind == kGZDown) {
-                rarr[tl] = r;
-                ndarr[tl] = nrm;
-                continue;
-            }
-        }
-        if (kind == kGAll || kind == kGZUp) {
-            if (kind == kGZUp) nrm = 0.0;
-            // rows above r: z_k = -(E_{k-1} / D-_k) z_{k-1}, segments upward
-            double z = 1.0;
-            double cbn = (S > 1) ? cpB[at(1)] : 0.0;
-            for (int s = 0; s < S; s++) {
+        };
+        // rows above r in segments s_lo up to s_hi: z_k = -(E_{k-1} / D-_k) z_{k-1}
+        auto walk_up = [&](int s_lo, int s_hi, double& z, double& nrm) {
+            double cbn = (s_lo + 1 < S) ? cpB[at(s_lo + 1)] : 0.0;
+            for (int s = s_lo; s <= s_hi; s++) {
                 const int lo = s * kSeg, len = min(kSeg, n - lo);
                 const double cbc = cbn;
                 if (s + 2 < S) cbn = cpB[at(s + 2)];
@@ -382,7 +447,128 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                     if (j == 0 && lo > r) zc[at(s)] = z;
                 }
             }
-            if (kind == kGZUp) {
+        };
+        // A long matrix's scan (kGZDown, kGZUp): each lane's twist segment is
+        // walked row by row (the wave takes its lanes' twist segments one
+        // after another), the segments beyond it are crossed with their
+        // transfers, kScanBlk at a time with the next block's loads in flight.
+        // An unusable transfer (Q < 0) sends the wave back to the row walk.
+        constexpr int kScanBlk = 8;
+        double nrm = 1.0;
+        if (kind == kGAll || kind == kGZDown) {
+            const int smax = wave_max(sr);
+            double z = 1.0;
+            if (kind == kGAll) {
+                walk_down(smax, 0, z, nrm);
+            } else {
+                bool todo = true;
+                for (;;) {
+                    const int s = wave_max(todo ? sr : -1);
+                    if (s < 0) break;
+                    double zs = 1.0, ns = 1.0;
+                    walk_down(s, s, zs, ns);
+                    if (todo && sr == s) {
+                        z = zs;
+                        nrm = ns;
+                        todo = false;
+                    }
+                }
+                double pb[kScanBlk], qb[kScanBlk], pn[kScanBlk], qn[kScanBlk];
+                auto fetch = [&](int s0, double(&p)[kScanBlk], double(&q)[kScanBlk]) {
+#pragma unroll
+                    for (int i = 0; i < kScanBlk; i++) {
+                        const int s = max(s0 - i, 0);
+                        p[i] = tPD[at(s)];
+                        q[i] = tQD[at(s)];
+                    }
+                };
+                bool bad = false;
+                fetch(smax - 1, pb, qb);
+                for (int s0 = smax - 1; s0 >= 0; s0 -= kScanBlk) {
+                    fetch(s0 - kScanBlk, pn, qn);
+#pragma unroll
+                    for (int i = 0; i < kScanBlk; i++) {
+                        const int s = s0 - i;
+                        if (s >= 0 && s < sr) {
+                            nrm = fma(z, z * qb[i], nrm);
+                            z = pb[i] * z;
+                            zc[at(s)] = z;
+                            bad = bad || qb[i] < 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < kScanBlk; i++) {
+                        pb[i] = pn[i];
+                        qb[i] = qn[i];
+                    }
+                }
+                if (__any(bad)) {
+                    z = 1.0;
+                    nrm = 1.0;
+                    walk_down(smax, 0, z, nrm);
+                }
+                rarr[tl] = r;
+                ndarr[tl] = nrm;
+                continue;
+            }
+        }
+        if (kind == kGAll || kind == kGZUp) {
+            if (kind == kGZUp) nrm = 0.0;
+            const int smin = -wave_max(-sr);
+            double z = 1.0;
+            if (kind == kGAll) {
+                walk_up(smin, S - 1, z, nrm);
+            } else {
+                bool todo = true;
+                for (;;) {
+                    const int s = -wave_max(todo ? -sr : -S);
+                    if (s >= S) break;
+                    double zs = 1.0, ns = 0.0;
+                    walk_up(s, s, zs, ns);
+                    if (todo && sr == s) {
+                        z = zs;
+                        nrm = ns;
+                        todo = false;
+                    }
+                }
+                double ub[kScanBlk], ab[kScanBlk], qb[kScanBlk];
+                double un[kScanBlk], an[kScanBlk], qn[kScanBlk];
+                auto fetch = [&](int s0, double(&u)[kScanBlk], double(&a)[kScanBlk],
+                                 double(&q)[kScanBlk]) {
+#pragma unroll
+                    for (int i = 0; i < kScanBlk; i++) {
+                        const int s = min(s0 + i, S - 1);
+                        u[i] = tU0[at(s)];
+                        a[i] = tAU[at(s)];
+                        q[i] = tQU[at(s)];
+                    }
+                };
+                bool bad = false;
+                fetch(smin + 1, ub, ab, qb);
+                for (int s0 = smin + 1; s0 < S; s0 += kScanBlk) {
+                    fetch(s0 + kScanBlk, un, an, qn);
+#pragma unroll
+                    for (int i = 0; i < kScanBlk; i++) {
+                        const int s = s0 + i;
+                        if (s < S && s > sr) {
+                            nrm = fma(z, z * qb[i], nrm);
+                            zc[at(s)] = ub[i] * z;
+                            z = ab[i] * z;
+                            bad = bad || qb[i] < 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < kScanBlk; i++) {
+                        ub[i] = un[i];
+                        ab[i] = an[i];
+                        qb[i] = qn[i];
+                    }
+                }
+                if (__any(bad)) {
+                    z = 1.0;
+                    nrm = 0.0;
+                    walk_up(smin, S - 1, z, nrm);
+                }
                 nuarr[tl] = nrm;
                 continue;
             }

@@ -164,3 +164,90 @@ def test_differentiated_workload_shapes(gpu):
         rd, re, _ = dense_grad(d, e, g[o:o + n])
         assert np.max(np.abs(gd[o:o + n] - rd)) <= 1e-10 * np.max(np.abs(g))
         assert np.max(np.abs(ge[o:o + n - 1] - re)) <= 1e-10 * np.max(np.abs(g))
+
+
+def unit_grads(torch, d, e, idx):
+    """dD, dE of lambda_i alone (g = unit vector e_i) for each i in idx."""
+    from brdc.autograd import grad_single
+    dev = torch.device("cuda:0")
+    dt, et = torch.tensor(d, device=dev), torch.tensor(e, device=dev)
+    import brdc
+    w, info = brdc.dstebr(d, e)
+    assert info == 0
+    wt = torch.tensor(w, device=dev)
+    out = []
+    for i in idx:
+        gi = torch.zeros(d.size, dtype=torch.float64, device=dev)
+        gi[i] = 1.0
+        dd, de = grad_single(dt, et, wt, gi)
+        out.append((dd.cpu().numpy(), de.cpu().numpy()))
+    return w, out
+
+
+@pytest.mark.parametrize("n", [9000, 16384])
+def test_long_matrix_unit_grads_toeplitz_closed_form(gpu, n):
+    """Matrices long enough for the multi-launch backward (twist search over
+    row ranges, z crossed by segment transfers).  Toeplitz (2, 1): the k-th
+    eigenvector is v(j) = sqrt(2/(n+1)) sin(j k pi/(n+1)), spread over every
+    row, so g = e_i must give dD = v^2 and dE = 2 v(j) v(j+1) everywhere."""
+    d, e = np.full(n, 2.0), np.full(n - 1, 1.0)
+    idx = [100, n // 3, n // 2, n - 101]
+    _, res = unit_grads(gpu, d, e, idx)
+    j = np.arange(1, n + 1)
+    for i, (dd, de) in zip(idx, res):
+        k = n - i  # ascending order: the i-th smallest has k = n - i
+        v = np.sqrt(2.0 / (n + 1)) * np.sin(j * k * np.pi / (n + 1))
+        assert np.all(np.isfinite(dd)) and np.all(np.isfinite(de))
+        assert np.max(np.abs(dd - v * v)) <= 1e-9 * np.max(v * v), i
+        assert np.max(np.abs(de - 2 * v[:-1] * v[1:])) <= 1e-9 * np.max(v * v), i
+
+
+def test_long_matrix_unit_grads_match_mrrr_vectors(gpu):
+    """A normal random n = 12000 matrix (long path): v_i^2 from the backward
+    against the eigenvector of scipy's eigh_tridiagonal (LAPACK MRRR) for the
+    same index, to a bound of 50 eps |T| / gap_i."""
+    from scipy.linalg import eigh_tridiagonal
+    n = 12000
+    d, e = generate("normal", n, seed_for(5, n, 7))
+    idx = [0, 17, n // 4, n // 2 + 3, n - 1]
+    w, res = unit_grads(gpu, d, e, idx)
+    tn = np.max(np.abs(d)) + 2 * np.max(np.abs(e))
+    for i, (dd, de) in zip(idx, res):
+        lam, V = eigh_tridiagonal(d, e, select="i", select_range=(i, i))
+        v = V[:, 0]
+        gap = min(abs(w[i] - w[i - 1]) if i > 0 else np.inf,
+                  abs(w[i + 1] - w[i]) if i + 1 < n else np.inf)
+        tol = max(1e-11, 50 * np.finfo(float).eps * tn / gap)
+        assert abs(lam[0] - w[i]) <= 1e-12 * tn
+        assert np.max(np.abs(dd - v * v)) <= tol, (i, gap)
+        assert np.max(np.abs(de - 2 * v[:-1] * v[1:])) <= 2 * tol, (i, gap)
+
+
+def test_batched_long_matrix_grad_equals_single(gpu):
+    """A batch holding a long matrix (n = 9000, multi-launch backward) beside
+    short ones: the long matrix's batched gradient equals its single-matrix
+    gradient bit for bit, and every matrix obeys sum_j dD_j = sum_i g_i."""
+    import brdc
+    from brdc.autograd import eigvalsh_tridiagonal_batched, grad_single
+    torch = gpu
+    dev = torch.device("cuda:0")
+    sizes = [300, 9000, 1024, 65]
+    mats = [generate("uniform", n, seed_for(11, n, b)) for b, n in enumerate(sizes)]
+    d_cat, e_cat, off = pack_batch(mats)
+    bs = brdc.BatchSolver(sizes, device=dev)
+    dt = torch.tensor(d_cat, device=dev, requires_grad=True)
+    et = torch.tensor(e_cat, device=dev, requires_grad=True)
+    g = torch.tensor(np.random.default_rng(9).standard_normal(d_cat.size), device=dev)
+    w = eigvalsh_tridiagonal_batched(dt, et, bs)
+    (w * g).sum().backward()
+    gd, ge, wn = dt.grad.cpu().numpy(), et.grad.cpu().numpy(), w.detach()
+    for b, (d, e) in enumerate(mats):
+        o, n = int(off[b]), sizes[b]
+        gb = g[o:o + n].cpu().numpy()
+        assert abs(gd[o:o + n].sum() - gb.sum()) <= 1e-11 * np.sum(np.abs(gb))
+    o, n = int(off[1]), sizes[1]
+    d, e = mats[1]
+    sd, se = grad_single(torch.tensor(d, device=dev), torch.tensor(e, device=dev),
+                         wn[o:o + n].contiguous(), g[o:o + n].contiguous())
+    np.testing.assert_array_equal(gd[o:o + n], sd.cpu().numpy())
+    np.testing.assert_array_equal(ge[o:o + n - 1], se.cpu().numpy())
