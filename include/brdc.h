@@ -176,6 +176,10 @@ int brdc_batch_plan_stats(const brdc_batch_plan *plan, int *out, int nout);
  * 256-byte aligned) holds per-wave pivot arrays and per-task partial sums;
  * the sums over tasks are added in a fixed order, so results are
  * reproducible.  Asynchronous on stream; returns a status for the launch.
+ * For n >= 8192 the eigenvectors are crossed segment by segment with stored
+ * transfers; a transfer whose partial products exceed 2^400 is replaced by a
+ * walk of the rows.  The environment variable BRDC_GRAD_SCAN_LIMIT (read at
+ * each call, a value in [0, 2^400]) lowers that bound; 0 walks every row.
  */
 int brdc_dstebr_grad_workspace(int n, size_t *bytes);
 int brdc_dstebr_grad_device(int n, const double *d_dev, const double *e_dev,

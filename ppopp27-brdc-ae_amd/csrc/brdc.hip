@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <climits>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -520,6 +521,17 @@ GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles,
 // twist search; z; contributions), then the per-matrix sums.  items: the
 // four item lists back to back (count[l] each), or nullptr for one matrix,
 // whose items follow from the task count and parts.
+// Bound on a segment transfer's partial products of z (kScanBig);
+// BRDC_GRAD_SCAN_LIMIT lowers it, and 0 sends every long-matrix z scan
+// through the row walk, so the two forms can be checked against each other.
+double grad_scan_limit() {
+    const char* s = std::getenv("BRDC_GRAD_SCAN_LIMIT");
+    if (!s || !*s) return kScanBig;
+    char* end = nullptr;
+    const double v = std::strtod(s, &end);
+    return (end != s && v >= 0.0 && v <= kScanBig) ? v : kScanBig;
+}
+
 int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const long long* poff,
                  const long long* cpoff, const int4* items, const int count[4], int parts,
                  int ntask, long long cp_doubles, long long partial_doubles, const double* d,
@@ -535,6 +547,7 @@ int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const 
     double* part = at<double>(scratch, G.partial);
     const int n_one = moff ? 0 : rows;
     if (ntask <= 0) return BRDC_STATUS_SUCCESS;
+    const double scan_big = grad_scan_limit();
     size_t base = 0;
     for (int l = 0; l < 4; l++) {
         const int cnt = count[l];
@@ -544,7 +557,7 @@ int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const 
                                dim3(kWave * kGradWaves), 0, st, l + 1,
                                items ? items + base : (const int4*)nullptr, cnt, parts, nmat, moff,
                                tprefix, poff, cpoff, n_one, d, e, w, g, cp, rarr, ndarr, nuarr,
-                               garr, rgarr, part);
+                               garr, rgarr, part, scan_big);
             BRDC_LAUNCH_CHECK();
         }
         base += (size_t)std::max(cnt, 0);

@@ -45,7 +45,8 @@ __host__ __device__ inline long long grad_cp_slot(int n) {
     return (grad_parts(n) > 1 ? 8 : 3) * (segs + 1) * 64;
 }
 // partial products of z above this bound mark a segment's transfer as unusable
-// (their squares and sums stay finite below it)
+// (their squares and sums stay finite below it); the kernel takes the bound
+// as an argument (scan_big) so that a smaller one can force the row walk
 constexpr double kScanBig = 0x1p400;
 
 // Task t of a batch: matrix b with tprefix[b] <= t < tprefix[b+1], first
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     const double* __restrict__ d, const double* __restrict__ e, const double* __restrict__ w,
     const double* __restrict__ g, double* __restrict__ cpbuf, int* __restrict__ rarr,
     double* __restrict__ ndarr, double* __restrict__ nuarr, double* __restrict__ garr,
-    int* __restrict__ rgarr, double* __restrict__ partial) {
+    int* __restrict__ rgarr, double* __restrict__ partial, double scan_big) {
     __shared__ double stage[kGradWaves][2][kWin];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = uniform((int)(threadIdx.x / kWave));
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         // D+_k, and QD = the segment's sum of (z_k / z_{lo+16})^2.  Up (rows
         // above it): z_lo = U0 z_{lo-1} and z_{lo+15} = AU z_{lo-1} with the
         // factors g_k = -E_{k-1} / D-_k, and QU = the sum of (z_k / z_{lo-1})^2.
-        // A partial product above kScanBig (or not finite) stores Q = -1, and
+        // A partial product above scan_big (or not finite) stores Q = -1, and
         // the scan walks the rows instead.
         auto gamma_scan = [&](int sa, int sb, double& gbest, int& rbest, bool store_b,
                               bool store_t) {
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                     for (int j = kSeg - 1; j >= 0; j--) {
                         pd *= -(ev[j] * rcp_chain(P[j]));
                         qd = fma(pd, pd, qd);
-                        okd = okd && fabs(pd) <= kScanBig;
+                        okd = okd && fabs(pd) <= scan_big;
                     }
                     double au = 1.0, qu = 0.0, u0 = 1.0;
                     bool oku = true;
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                         if (j == 0) u0 = au;
                         const bool in = j < len;
                         qu = in ? fma(au, au, qu) : qu;
-                        oku = oku && (!in || fabs(au) <= kScanBig);
+                        oku = oku && (!in || fabs(au) <= scan_big);
                     }
                     tPD[at(s)] = pd;
                     tQD[at(s)] = okd ? qd : -1.0;

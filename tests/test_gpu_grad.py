@@ -251,3 +251,25 @@ def test_batched_long_matrix_grad_equals_single(gpu):
                          wn[o:o + n].contiguous(), g[o:o + n].contiguous())
     np.testing.assert_array_equal(gd[o:o + n], sd.cpu().numpy())
     np.testing.assert_array_equal(ge[o:o + n - 1], se.cpu().numpy())
+
+
+@pytest.mark.parametrize("kind", ["normal", "toeplitz"])
+def test_long_matrix_transfer_scan_equals_row_walk(gpu, kind, monkeypatch):
+    """The long-matrix z scan crosses segments with stored transfers and walks
+    rows only where a transfer is unusable.  BRDC_GRAD_SCAN_LIMIT = 0 marks
+    every transfer unusable, so the same gradient comes from walking every
+    row: the two forms agree to rounding (n = 9000, random g)."""
+    n = 9000
+    d, e = generate(kind, n, seed_for(5, n, 3))
+    g = np.random.default_rng(12).standard_normal(n)
+    gd, ge, _ = gpu_grad(gpu, d, e, g)
+    monkeypatch.setenv("BRDC_GRAD_SCAN_LIMIT", "0")
+    wd, we, _ = gpu_grad(gpu, d, e, g)
+    monkeypatch.delenv("BRDC_GRAD_SCAN_LIMIT")
+    scale = np.max(np.abs(g))
+    assert np.all(np.isfinite(wd)) and np.all(np.isfinite(we))
+    assert np.max(np.abs(gd - wd)) <= 1e-12 * scale
+    assert np.max(np.abs(ge - we)) <= 1e-12 * scale
+    # the two forms round differently: identical bits would mean the limit
+    # never reached the kernel
+    assert not np.array_equal(gd, wd)
