@@ -11,7 +11,7 @@ leg for the others, and the trace / Frobenius invariants for all).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 49152]
        [--no-cpu-baseline] [--leaf L] [--eager] [--no-batched] [--no-grad]
-       [--no-vendor] [--no-compare] [--only batched|differentiated|vendor]
+       [--no-vendor] [--no-compare] [--only batched|differentiated|vendor|config3]
 
 Besides the main line's fields, the JSON line carries the batched workload
 (2048 matrices, solves/s), the differentiated workload (forward + backward
@@ -314,6 +314,55 @@ def bench_differentiated(steps, warmup, compare=True):
             "pass": bool(worst_id < 1e-10 and worst_dense < 1e-10)}
 
 
+def bench_config3_and_host_entry(reps=5):
+    """The largest configured size (N = 65536 normal, one solve): device ms per
+    solve as a graph replay with the inputs resident, its workspace bytes, and
+    the error against the Toeplitz-free invariants; then the host-array entry
+    brdc_dstebr on the N = 49152 normal matrix, whose time includes the H2D
+    copy of D and E, the device workspace allocation and the D2H copy of the
+    eigenvalues (the PCIe-inclusive rate, reported beside the value, never as
+    it)."""
+    import torch
+    import brdc
+    from brdc.generators import generate, seed_for
+    dev = torch.device("cuda:0")
+    n = 65536
+    d, e = generate("normal", n, seed_for(3, n))
+    dt, et = torch.from_numpy(d).to(dev), torch.from_numpy(e).to(dev)
+    gs = brdc.GraphSolve(brdc.Solver(n, device=dev), dt, et)
+    for _ in range(2):
+        gs.run()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        gs.run()
+    b.record()
+    torch.cuda.synchronize()
+    w = gs.w.cpu().numpy()
+    scale = np.max(np.abs(w))
+    inv = max(abs(w.sum() - d.sum()) / (scale * n),
+              abs((w * w).sum() - (d * d).sum() - 2 * (e * e).sum()) / (scale ** 2 * n))
+    rec = {"workload": "N=65536, D,E ~ N(0,1), one solve (hipGraph replay, inputs in HBM)",
+           "ms_per_solve": round(a.elapsed_time(b) / reps, 4),
+           "workspace_bytes": int(brdc.workspace_bytes(n)),
+           "info": int(gs.solver.info.item()), "invariants_rel": float(inv),
+           "sorted": bool(np.all(np.diff(w) >= 0))}
+    n2 = 49152
+    d2, e2 = generate("normal", n2, seed_for(2, n2))
+    brdc.dstebr(d2, e2)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, info2 = brdc.dstebr(d2, e2)
+        times.append((time.perf_counter() - t0) * 1e3)
+    rec["host_entry"] = {"workload": "brdc_dstebr (host arrays) on the N=49152 normal matrix: "
+                                     "H2D of D and E, device workspace allocation, solve, D2H of "
+                                     "the eigenvalues", "ms_per_solve_median": round(
+                                         float(np.median(times)), 3), "info": int(info2)}
+    return rec
+
+
 def bench_vendor(n=4096):
     """rocSOLVER STERF and STEDC (values only) on the same GPU for the
     N=4096 normal matrix: a reported vendor baseline.  Both are sequential
@@ -370,7 +419,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the per-matrix GPU comparisons of the batched and "
                          "differentiated sections")
-    ap.add_argument("--only", choices=("batched", "differentiated", "vendor"), default=None,
+    ap.add_argument("--only", choices=("batched", "differentiated", "vendor", "config3"),
+                    default=None,
                     help="run only this section and print its record as the JSON line "
                          "(for profiling one workload at a time)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -391,7 +441,7 @@ def main():
                                                 leaf=args.leaf, compare=not args.no_compare),
                "differentiated": lambda: bench_differentiated(args.steps, args.warmup,
                                                               compare=not args.no_compare),
-               "vendor": bench_vendor}[args.only]()
+               "vendor": bench_vendor, "config3": bench_config3_and_host_entry}[args.only]()
         print(json.dumps({"section": args.only, **rec}), flush=True)
         return
 
@@ -572,6 +622,7 @@ def main():
                                                       compare=not args.no_compare)
     if not args.no_vendor:
         line["vendor_baseline"] = bench_vendor()
+        line["config3_and_host_entry"] = bench_config3_and_host_entry()
     print(json.dumps(line), flush=True)
 
 

@@ -96,6 +96,22 @@ def test_large_sizes_closed_form_and_invariants(gpu, n):
         assert abs((w * w).sum() - (d * d).sum() - 2 * (e * e).sum()) <= 1e-12 * scale ** 2 * n
 
 
+def test_n65536_normal_vs_oracle_and_workspace(gpu, orc):
+    """The largest configured size, N = 65536 normal: against the oracle to
+    1e-10, the trace and Frobenius invariants, and its device workspace
+    (reported by the query) a small fraction of one GPU's HBM."""
+    import brdc
+    n = 65536
+    d, e = generate("normal", n, seed_for(3, n))
+    ref, info_ref = orc.dstebr(d, e)
+    w, info, _ = solve(gpu, d, e)
+    assert info == 0 and info_ref == 0 and rel_err(w, ref) <= TOL
+    scale = np.max(np.abs(w))
+    assert abs(w.sum() - d.sum()) <= 1e-12 * scale * n
+    assert abs((w * w).sum() - (d * d).sum() - 2 * (e * e).sum()) <= 1e-12 * scale ** 2 * n
+    assert brdc.workspace_bytes(n) < 1 << 30
+
+
 def test_n49152_normal_vs_oracle(gpu, orc):
     n = 49152
     d, e = generate("normal", n, seed_for(2, n))
