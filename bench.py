@@ -7,7 +7,9 @@ Toeplitz (D = 2, E = 1) and clustered (8 centres, 1e-8 spread, E ~ 1e-6 U(0,1))
 
 Each solve is checked (<= 1e-10 normwise vs an independent reference: the
 closed form for Toeplitz, the CPU oracle's eigenvalues from the cpu_baseline
-leg for the others, and the trace / Frobenius invariants for all).
+leg for the others, and the trace / Frobenius invariants for all); the CPU
+leg also bisects 8 eigenvalues per generator in binary128 and reports the
+GPU's error there as max |dw| / max(1, max |w|).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 49152]
        [--no-cpu-baseline] [--leaf L] [--eager] [--no-batched] [--no-grad]
@@ -103,6 +105,32 @@ def cpu_baseline(mats, budget_s=25.0):
                       (cores, len(mats["normal"][0]), ", ".join(g for g in GENS if g != "toeplitz"),
                        "" if "toeplitz" in per else " skipped (over budget)"))}
     return rec, eigs
+
+
+def quad_probes(mats, outs, probes=8):
+    """High-precision check, part of the CPU leg: at evenly spread probe
+    indices of every generator, eigenvalue k by binary128 Sturm bisection
+    (oracle.eig_quad_probe, the role of the reference's MPFR probe), and the
+    GPU's error there as max |w_k - lambda_k| / max(1, max |lambda|) (the
+    reference's MPFR error measure).  The probes run in threads (the C call
+    releases the interpreter lock)."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = []
+    for g in GENS:
+        d, e = mats[g]
+        n = d.size
+        for k in sorted({int(round(i * (n - 1) / (probes - 1))) for i in range(probes)}):
+            jobs.append((g, k, d, e, float(outs[g][k])))
+    workers = min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        res = list(ex.map(lambda j: oracle.eig_quad_probe(j[2], j[3], j[1], j[4]), jobs))
+    out = {}
+    for (g, k, d, e, wk), r in zip(jobs, res):
+        scale = max(1.0, float(np.max(np.abs(outs[g]))))
+        err = float("inf") if r is None else abs((wk - r[0]) - r[1]) / scale
+        out.setdefault(g, {})[str(k)] = err
+    return {g: {"probes": v, "max": max(v.values())} for g, v in out.items()}
 
 
 def copy_probe(mib):
@@ -538,9 +566,10 @@ def main():
 
     # accuracy
     acc = {}
-    cpu_rec, cpu_eigs = (None, {})
+    cpu_rec, cpu_eigs, qp = (None, {}, None)
     if not args.no_cpu_baseline:
         cpu_rec, cpu_eigs = cpu_baseline(mats)
+        qp = quad_probes(mats, outs)
     for g in GENS:
         d, e = mats[g]
         w = outs[g]
@@ -551,6 +580,8 @@ def main():
             rec["vs_closed_form"] = rel_err(w, ref)
         if g in cpu_eigs:
             rec["vs_cpu_oracle"] = rel_err(w, cpu_eigs[g])
+        if qp and g in qp:
+            rec["vs_binary128_probes"] = qp[g]["max"]
         acc[g] = rec
     worst = max(max(v for k, v in r.items() if k.startswith("vs_")) if any(
         k.startswith("vs_") for k in r) else 0.0 for r in acc.values())
@@ -605,7 +636,8 @@ def main():
                    "per_generator_ms_eager_launch": {g: round(v, 4)
                                                      for g, v in per_gen_eager.items()},
                    "workspace_bytes": solver.bytes},
-        "accuracy": {"pass": ok, "worst_rel_err": worst, "per_generator": acc},
+        "accuracy": {"pass": ok, "worst_rel_err": worst, "per_generator": acc,
+                     "binary128_probes": qp},
         "roofline": roof,
         "kernel_breakdown_ms_per_step": {k: round(v["ms"], 4) for k, v in prof["kernels"].items()},
         "cpu_baseline": cpu_rec,

@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_int, _ip]
         L.brdc_oracle_lamrg.restype = None
         L.brdc_oracle_wprod_nthreads.argtypes = []
+        L.brdc_oracle_eig_quad.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -169,6 +171,41 @@ def sterf(d, e):
     e_buf[: max(n - 1, 0)] = np.asarray(e, dtype=np.float64)[: max(n - 1, 0)]
     info = lib().brdc_oracle_sterf(n, _p(d), _p(e_buf))
     return d, info
+
+
+def eig_quad(d, e, k: int, lo: float, hi: float, steps: int = 120):
+    """Eigenvalue k (0-based, ascending) by binary128 Sturm bisection from the
+    bracket [lo, hi]; returns (hi, lo) of a double-double, or None when the
+    bracket does not hold eigenvalue k."""
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    n = d.shape[0]
+    e_buf = np.zeros(max(n, 1))
+    e_buf[: max(n - 1, 0)] = np.asarray(e, dtype=np.float64)[: max(n - 1, 0)]
+    out = np.zeros(2)
+    rc = lib().brdc_oracle_eig_quad(n, _p(d), _p(e_buf), int(k), float(lo), float(hi),
+                                    int(steps), _p(out))
+    return None if rc else (float(out[0]), float(out[1]))
+
+
+def eig_quad_probe(d, e, k: int, approx: float, steps: int = 70):
+    """Eigenvalue k near approx: a bracket approx -/+ r with r doubled from
+    1e-12 |T| until it holds the eigenvalue (the whole Gershgorin interval at
+    most), then steps binary128 halvings."""
+    d = np.asarray(d, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
+    ea = np.abs(e)
+    rad = np.zeros(d.size)
+    rad[:-1] += ea
+    rad[1:] += ea
+    glo, ghi = float(np.min(d - rad)), float(np.max(d + rad))
+    tn = max(abs(glo), abs(ghi), 1e-300)
+    r = 1e-12 * tn
+    while True:
+        lo, hi = max(approx - r, glo - r), min(approx + r, ghi + r)
+        res = eig_quad(d, e, k, lo, hi, steps)
+        if res is not None or (lo <= glo - r / 2 and hi >= ghi + r / 2):
+            return res
+        r *= 4.0
 
 
 def lamrg(a, n1: int, n2: int, s1: int = 1, s2: int = 1):

@@ -30,6 +30,7 @@
  */
 #include "brdc_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1564,4 +1565,42 @@ int brdc_oracle_dstebr(int n, double *d, double *e, double *work, int lwork,
 {
     return brdc_oracle_dstebr_ex(n, d, e, work, lwork, iwork, liwork, 0, 0,
                                  NULL, 0, NULL);
+}
+
+/* ---- binary128 Sturm bisection (high-precision checker) ---------------- */
+
+/* number of eigenvalues of (d, e) below x: negative pivots of T - x I */
+static int sturm_count_q(int n, const double *d, const double *e, __float128 x,
+                         __float128 pivmin) {
+    int neg = 0;
+    __float128 q = (__float128)d[0] - x;
+    for (int i = 0;; i++) {
+        if ((q < 0 ? -q : q) < pivmin) q = -pivmin;
+        if (q < 0) neg++;
+        if (i + 1 >= n) break;
+        const __float128 ei = (__float128)e[i];
+        q = ((__float128)d[i + 1] - x) - ei * ei / q;
+    }
+    return neg;
+}
+
+int brdc_oracle_eig_quad(int n, const double *d, const double *e, int k, double lo,
+                         double hi, int steps, double *out) {
+    if (n < 1 || k < 0 || k >= n || !(lo < hi)) return -1;
+    double emax = 0.0;
+    for (int i = 0; i + 1 < n; i++) emax = fmax(emax, fabs(e[i]));
+    /* DLAEBZ's pivmin: safe minimum times max(1, max e^2), in binary128 */
+    const __float128 pivmin = (__float128)DBL_MIN * (__float128)fmax(1.0, emax * emax);
+    __float128 a = lo, b = hi;
+    if (sturm_count_q(n, d, e, a, pivmin) > k || sturm_count_q(n, d, e, b, pivmin) <= k)
+        return -1;
+    for (int s = 0; s < steps; s++) {
+        const __float128 m = (a + b) / 2;
+        if (sturm_count_q(n, d, e, m, pivmin) > k) b = m;
+        else a = m;
+    }
+    const __float128 m = (a + b) / 2;
+    out[0] = (double)m;
+    out[1] = (double)(m - (__float128)out[0]);
+    return 0;
 }

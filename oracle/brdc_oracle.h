@@ -78,6 +78,19 @@ int brdc_oracle_leaf(int n, double *d, double *e, double *blo, double *bhi);
  * ascending output. */
 int brdc_oracle_sterf(int n, double *d, double *e);
 
+/* High-precision checker (the role of the reference's MPFR probe,
+ * run_mpfr_main_families_probe_job.sh:340-416, and of the error column of
+ * README.md:70-75): eigenvalue k (0-based, ascending) of the symmetric
+ * tridiagonal (d, e) by bisection on Sturm counts evaluated in binary128
+ * (the count of negative pivots of the LDL^T factorization of T - x I, a
+ * pivot below pivmin in magnitude replaced by -pivmin as in DLAEBZ).  The
+ * bracket [lo, hi] must hold the eigenvalue (count(lo) <= k < count(hi));
+ * steps halvings follow.  Returns the midpoint of the last bracket as the
+ * double-double out[0] + out[1]; 0 on success, -1 if the bracket does not
+ * hold eigenvalue k. */
+int brdc_oracle_eig_quad(int n, const double *d, const double *e, int k, double lo,
+                         double hi, int steps, double *out);
+
 /* DLAMRG: merge permutation of two sorted sub-lists (0-based output). */
 void brdc_oracle_lamrg(int n1, int n2, const double *a, int s1, int s2,
                        int *index);

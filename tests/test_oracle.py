@@ -253,3 +253,38 @@ def test_lamrg_brute_force(orc):
                     pos = [p for p in idx if a[p] == v]
                     side = [p >= n1 for p in pos]
                     assert side == sorted(side)
+
+
+def test_eig_quad_closed_forms(orc):
+    """The binary128 Sturm bisection against closed forms evaluated in
+    40-digit arithmetic (mpmath): tridiag(1, 0, 1), eigenvalues 2 cos(k pi /
+    (n + 1)), exactly representable entries, so the only error is the
+    bisection's own (about 1e-33); and the Kac matrix D = 0, E_i = sqrt(i (n -
+    i)) (eigenvalues -(n-1), -(n-3), ..., n-1) to the rounding of its E."""
+    import mpmath
+    mpmath.mp.dps = 40
+    n = 300
+    d, e = np.zeros(n), np.ones(n - 1)
+    for k in (0, 1, 77, 150, 298, 299):
+        exact = 2 * mpmath.cos((n - k) * mpmath.pi / (n + 1))
+        hi, lo = orc.eig_quad_probe(d, e, k, float(exact) + 1e-9)
+        assert abs(mpmath.mpf(hi) + mpmath.mpf(lo) - exact) < 1e-30
+    m = 64
+    ek = np.sqrt(np.arange(1, m) * (m - np.arange(1, m)))
+    for k in (0, 5, 31, 63):
+        hi, lo = orc.eig_quad_probe(np.zeros(m), ek, k, -(m - 1) + 2 * k + 0.3)
+        assert abs(hi - (-(m - 1) + 2 * k)) <= 1e-13 * m
+
+
+def test_eig_quad_agrees_with_oracle_and_rejects_bad_brackets(orc):
+    """A random normal matrix: binary128 probes agree with the double BR
+    oracle to its rounding, and a bracket that misses eigenvalue k is
+    reported (None) rather than bisected."""
+    n = 500
+    d, e = generate("normal", n, seed_for(9, n))
+    w, info = orc.dstebr(d, e)
+    assert info == 0
+    for k in (0, 123, 250, 499):
+        hi, lo = orc.eig_quad_probe(d, e, k, w[k])
+        assert abs(hi - w[k]) <= 1e-14 * np.max(np.abs(w))
+    assert orc.eig_quad(d, e, 250, w[300] - 1e-9, w[300] + 1e-9) is None
