@@ -580,6 +580,9 @@ def main():
             rec["vs_closed_form"] = rel_err(w, ref)
         if g in cpu_eigs:
             rec["vs_cpu_oracle"] = rel_err(w, cpu_eigs[g])
+            # the 2-norm form of tridiag_dc_bench.cu:747-757
+            rec["vs_cpu_oracle_2norm"] = float(np.linalg.norm(w - cpu_eigs[g]) /
+                                               max(np.linalg.norm(cpu_eigs[g]), 1e-300))
         if qp and g in qp:
             rec["vs_binary128_probes"] = qp[g]["max"]
         acc[g] = rec
