@@ -98,7 +98,24 @@ def cpu_baseline(mats, budget_s=25.0):
         per["toeplitz"] = 1e3 * (time.perf_counter() - t0)
         eigs["toeplitz"] = w
     timed = [per[g] for g in GENS if g in per]
+    # LAPACK DSTERF, the reference's acceptance oracle (smoke_br.f90:30), on
+    # one core: O(N^2), so timed on a bounded sample, the N = 16384 normal
+    # matrix (scipy's OpenBLAS, whose DSTERF is sequential)
+    sterf = None
+    try:
+        from scipy.linalg import lapack
+        from brdc.generators import generate, seed_for
+        ds, es = generate("normal", 16384, seed_for(2, 16384))
+        t0 = time.perf_counter()
+        _, info_s = lapack.dsterf(ds.copy(), es.copy())
+        sterf = {"value": round(1e3 * (time.perf_counter() - t0), 1), "unit": "ms/solve",
+                 "cores": 1, "kind": "lapack", "info": int(info_s),
+                 "sample": "LAPACK DSTERF (scipy's OpenBLAS) on the N=16384 normal matrix, "
+                           "one solve"}
+    except ImportError:
+        pass
     rec = {"value": float(np.mean(timed)), "unit": "ms/solve", "cores": cores, "kind": "port",
+           "lapack_dsterf": sterf,
            "per_generator_ms": {g: round(v, 3) for g, v in per.items()},
            "sample": ("oracle BR restatement (OpenMP, %d threads) on the same N=%d matrices: "
                       "median of up to 5 solves each for %s; one Toeplitz solve%s" %
