@@ -224,14 +224,15 @@ def test_long_matrix_unit_grads_match_mrrr_vectors(gpu):
 
 
 def test_batched_long_matrix_grad_equals_single(gpu):
-    """A batch holding a long matrix (n = 9000, multi-launch backward) beside
-    short ones: the long matrix's batched gradient equals its single-matrix
-    gradient bit for bit, and every matrix obeys sum_j dD_j = sum_i g_i."""
+    """A batch holding two long matrices (n = 9000 and 12000, multi-launch
+    backward) beside short ones: each long matrix's batched gradient equals its
+    single-matrix gradient bit for bit, and every matrix obeys sum_j dD_j =
+    sum_i g_i."""
     import brdc
     from brdc.autograd import eigvalsh_tridiagonal_batched, grad_single
     torch = gpu
     dev = torch.device("cuda:0")
-    sizes = [300, 9000, 1024, 65]
+    sizes = [300, 9000, 1024, 65, 12000]
     mats = [generate("uniform", n, seed_for(11, n, b)) for b, n in enumerate(sizes)]
     d_cat, e_cat, off = pack_batch(mats)
     bs = brdc.BatchSolver(sizes, device=dev)
@@ -245,21 +246,22 @@ def test_batched_long_matrix_grad_equals_single(gpu):
         o, n = int(off[b]), sizes[b]
         gb = g[o:o + n].cpu().numpy()
         assert abs(gd[o:o + n].sum() - gb.sum()) <= 1e-11 * np.sum(np.abs(gb))
-    o, n = int(off[1]), sizes[1]
-    d, e = mats[1]
-    sd, se = grad_single(torch.tensor(d, device=dev), torch.tensor(e, device=dev),
-                         wn[o:o + n].contiguous(), g[o:o + n].contiguous())
-    np.testing.assert_array_equal(gd[o:o + n], sd.cpu().numpy())
-    np.testing.assert_array_equal(ge[o:o + n - 1], se.cpu().numpy())
+    for b in (1, 4):  # the two long matrices, of different sizes
+        o, n = int(off[b]), sizes[b]
+        d, e = mats[b]
+        sd, se = grad_single(torch.tensor(d, device=dev), torch.tensor(e, device=dev),
+                             wn[o:o + n].contiguous(), g[o:o + n].contiguous())
+        np.testing.assert_array_equal(gd[o:o + n], sd.cpu().numpy())
+        np.testing.assert_array_equal(ge[o:o + n - 1], se.cpu().numpy())
 
 
-@pytest.mark.parametrize("kind", ["normal", "toeplitz"])
-def test_long_matrix_transfer_scan_equals_row_walk(gpu, kind, monkeypatch):
+@pytest.mark.parametrize("kind,n", [("normal", 9000), ("toeplitz", 9000), ("uniform", 8192)])
+def test_long_matrix_transfer_scan_equals_row_walk(gpu, kind, n, monkeypatch):
     """The long-matrix z scan crosses segments with stored transfers and walks
     rows only where a transfer is unusable.  BRDC_GRAD_SCAN_LIMIT = 0 marks
     every transfer unusable, so the same gradient comes from walking every
-    row: the two forms agree to rounding (n = 9000, random g)."""
-    n = 9000
+    row: the two forms agree to rounding (n = 9000, and n = 8192, the first
+    size on the long path; random g)."""
     d, e = generate(kind, n, seed_for(5, n, 3))
     g = np.random.default_rng(12).standard_normal(n)
     gd, ge, _ = gpu_grad(gpu, d, e, g)
