@@ -670,6 +670,9 @@ __global__ void k_grad_reduce(int rows, int nmat, const int* __restrict__ moff,
     }
     const int k = j - off;
     double sd = 0.0, se = 0.0;
+    // unrolled so that the loads of several tasks are in flight; the adds
+    // keep the task order
+#pragma unroll 8
     for (int t = 0; t < nt; t++) {
         const double* p = partial + po + (long long)t * 2 * n;
         sd += p[k];
