@@ -1609,32 +1609,49 @@ __global__ __launch_bounds__(kWave * W) void k_secular_mid(Geo g, int merges, in
             const int q0 = (int)(((long long)K * wave) / W);
             const int q1 = (int)(((long long)K * (wave + 1)) / W);
             double S = 0.0;
+            // eight poles (two quads) in pole order
+            auto batch8 = [&](const double2(&p)[8]) {
+#pragma unroll
+                for (int h = 0; h < 8; h += 4) {
+                    double x[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) x[q] = (p[h + q].x - org) - tau;
+                    const double u = x[0] * x[1], v = x[2] * x[3];
+                    const double ruv = rcp_stream(u * v);
+                    const double a = fma(p[h + 1].y, x[0], p[h].y * x[1]);
+                    const double b = fma(p[h + 3].y, x[2], p[h + 2].y * x[3]);
+                    S = fma(fma(b, u, a * v), ruv, S);
+                }
+            };
+            auto load8 = [&](double2(&p)[8], int j) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) p[q] = dz[j + q];
+            };
             auto plain = [&](int ja, int jb) {
                 int j = ja;
                 if (j + 8 <= jb) {
-                    // eight poles a step; the next eight are in flight meanwhile
-                    double2 nx[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) nx[q] = dz[j + q];
-                    for (; j + 8 <= jb; j += 8) {
-                        double2 p[8];
-#pragma unroll
-                        for (int q = 0; q < 8; q++) p[q] = nx[q];
-                        if (j + 16 <= jb) {
-#pragma unroll
-                            for (int q = 0; q < 8; q++) nx[q] = dz[j + 8 + q];
-                        }
-#pragma unroll
-                        for (int h = 0; h < 8; h += 4) {
-                            double x[4];
-#pragma unroll
-                            for (int q = 0; q < 4; q++) x[q] = (p[h + q].x - org) - tau;
-                            const double u = x[0] * x[1], v = x[2] * x[3];
-                            const double ruv = rcp_stream(u * v);
-                            const double a = fma(p[h + 1].y, x[0], p[h].y * x[1]);
-                            const double b = fma(p[h + 3].y, x[2], p[h + 2].y * x[3]);
-                            S = fma(fma(b, u, a * v), ruv, S);
-                        }
+                    // two buffers of eight poles in turn: a buffer's (scalar)
+                    // loads are waited for before the other's are issued, so
+                    // each batch's arithmetic overlaps the next batch's loads
+                    // without copies between the buffers (scalar loads return
+                    // out of order: any wait is a wait for all of them)
+                    double2 A[8], B[8];
+                    load8(A, j);
+                    for (; j + 16 <= jb; j += 16) {
+                        __builtin_amdgcn_sched_barrier(0);
+                        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): A is in
+                        load8(B, j + 8);
+                        __builtin_amdgcn_sched_barrier(0);   // B in flight before A's work
+                        batch8(A);
+                        __builtin_amdgcn_sched_barrier(0);
+                        __builtin_amdgcn_s_waitcnt(0xC07F);  // B is in
+                        load8(A, min(j + 16, jb - 8));       // in range; used if j + 24 <= jb
+                        __builtin_amdgcn_sched_barrier(0);
+                        batch8(B);
+                    }
+                    if (j + 8 <= jb) {  // A holds poles j .. j+7
+                        batch8(A);
+                        j += 8;
                     }
                 }
                 for (; j < jb; j++) {
