@@ -211,6 +211,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         const double gi = active ? g[off + i] : 0.0;
         // pivmin = safmin * max(1, max E^2) (DLARRD / DLAR1V)
         double emax = 0.0;
+#pragma unroll 8
         for (int k = lane; k < n - 1; k += kWave) emax = fmax(emax, fabs(e[off + k]));
         for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, kWave));
         const double pivmin = safmin * fmax(1.0, emax * emax);
