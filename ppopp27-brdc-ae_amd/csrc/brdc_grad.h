@@ -246,10 +246,21 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         auto window = [&](int lo, bool up) {
             if (lo - 1 >= wbase && lo + kSeg + 1 <= wbase + kWin) return;
             wbase = up ? lo - 1 : lo + kSeg + 1 - kWin;
-            for (int m = lane; m < kWin; m += kWave) {
-                const int row = wbase + m;
-                sD[m] = d[off + min(max(row, 0), n - 1)];
-                sE[m] = (row >= 0 && row <= n - 2) ? e[off + row] : 0.0;
+            // every load in flight before the first store (n >= 2 here, so
+            // the clamped E index is in range; its value is dropped outside)
+            constexpr int kPer = kWin / kWave;
+            double vd[kPer], ve[kPer];
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                const int row = wbase + lane + q * kWave;
+                vd[q] = d[off + min(max(row, 0), n - 1)];
+                const double x = e[off + min(max(row, 0), n - 2)];
+                ve[q] = (row >= 0 && row <= n - 2) ? x : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                sD[lane + q * kWave] = vd[q];
+                sE[lane + q * kWave] = ve[q];
             }
         };
         double dv[kSeg + 1], ev[kSeg], eprev;
