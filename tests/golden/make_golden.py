@@ -8,8 +8,9 @@ library family the reference links, cpu/README.md):
     vectors (the reference calls DLAED4 at dlaed4_br.f:59 and dlaed9_br.f:87-127)
 
 Inputs come from brdc.generators, so the fixtures also pin the generators.
-Usage:  python tests/golden/make_golden.py [reduced]
-        (``reduced``: only the DSTERF fixture of reduced_dense_49152.de.bin)
+Usage:  python tests/golden/make_golden.py [reduced | config2]
+        (``reduced``: only the DSTERF fixture of reduced_dense_49152.de.bin;
+        ``config2``: only the DSTERF fixtures of the N >= 16384 plain solves)
 """
 from __future__ import annotations
 
@@ -119,9 +120,33 @@ def reduced_dense_fixture():
     np.savez_compressed(os.path.join(HERE, "reduced_dense_49152_dsterf.npz"), w=w)
 
 
+CONFIG2_SIZES = (16384, 32768, 49152)
+
+
+def config2_fixture():
+    """DSTERF eigenvalues of the twelve plain-solve matrices: N in {16384,
+    32768, 49152} x {uniform, normal, toeplitz, clustered}, drawn with
+    ``generate(kind, n, seed_for(2, n))`` (the inputs the GPU tests and the
+    bench solve).  DSTERF is O(N^2) and sequential: about 4 minutes for all
+    twelve in one core."""
+    from scipy.linalg.lapack import dsterf
+    ev = {}
+    for n in CONFIG2_SIZES:
+        for kind in GENERATORS:
+            d, e = generate(kind, n, seed_for(2, n))
+            w, info = dsterf(d.copy(), e.copy())[:2]
+            assert info == 0
+            ev[f"{kind}_{n}"] = w
+            print(f"dsterf {kind} n={n} done", flush=True)
+    np.savez_compressed(os.path.join(HERE, "dsterf_config2.npz"), **ev)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "reduced":
         reduced_dense_fixture()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "config2":
+        config2_fixture()
         return
     L = openblas()
     from scipy.linalg.lapack import dsterf

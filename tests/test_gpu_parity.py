@@ -72,28 +72,14 @@ def test_matches_oracle_and_deflation_decisions(gpu, orc, kind, leaf):
     assert rel_err(w, ref) <= TOL
     st = s.merge_stats()
     # same tree, and the same number of non-deflated poles per merge except at
-    # the tolerance edge (leaf solvers differ: Jacobi here, QL in the oracle)
+    # the tolerance edge (both leaf solvers run QL, but their rotations round
+    # differently, so a pole at the deflation tolerance may go either way)
     ko = {(int(r["level"]), int(r["start"])): int(r["k"]) for r in st_ref}
     kg = {(int(r["level"]), int(r["start"])): int(r["k"]) for r in st}
     assert set(ko) == set(kg)
     diffs = [abs(ko[k] - kg[k]) for k in ko]
     assert sum(1 for x in diffs if x) <= max(2, len(diffs) // 20)
     assert max(diffs) <= 3
-
-
-@pytest.mark.parametrize("n", [16384, 49152])
-def test_large_sizes_closed_form_and_invariants(gpu, n):
-    # Toeplitz D=2, E=1: 2 + 2 cos(k pi / (n+1))
-    w, info, _ = solve(gpu, np.full(n, 2.0), np.full(n - 1, 1.0))
-    ref = np.sort(2.0 + 2.0 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
-    assert info == 0 and rel_err(w, ref) <= TOL
-    for kind in ("normal", "uniform", "clustered"):
-        d, e = generate(kind, n, seed_for(2, n))
-        w, info, _ = solve(gpu, d, e)
-        assert info == 0 and np.all(np.diff(w) >= 0)
-        scale = np.max(np.abs(w))
-        assert abs(w.sum() - d.sum()) <= 1e-12 * scale * n
-        assert abs((w * w).sum() - (d * d).sum() - 2 * (e * e).sum()) <= 1e-12 * scale ** 2 * n
 
 
 def test_n65536_normal_vs_oracle_and_workspace(gpu, orc):
@@ -110,14 +96,6 @@ def test_n65536_normal_vs_oracle_and_workspace(gpu, orc):
     assert abs(w.sum() - d.sum()) <= 1e-12 * scale * n
     assert abs((w * w).sum() - (d * d).sum() - 2 * (e * e).sum()) <= 1e-12 * scale ** 2 * n
     assert brdc.workspace_bytes(n) < 1 << 30
-
-
-def test_n49152_normal_vs_oracle(gpu, orc):
-    n = 49152
-    d, e = generate("normal", n, seed_for(2, n))
-    ref, info_ref = orc.dstebr(d, e)
-    w, info, _ = solve(gpu, d, e)
-    assert info == 0 and rel_err(w, ref) <= TOL
 
 
 def test_closed_form_families(gpu):

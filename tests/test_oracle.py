@@ -142,6 +142,18 @@ def test_dstebr_matches_dsterf_fixtures(orc):
             assert rel_err(w, g[f"{kind}_{n}"]) < 1e-13, (kind, n)
 
 
+@pytest.mark.parametrize("n", [16384, 32768, 49152])
+def test_dstebr_matches_dsterf_at_full_size(orc, n):
+    """The plain-solve matrices at full size (four generators,
+    seed_for(2, n)) against LAPACK DSTERF (tests/golden/dsterf_config2.npz)."""
+    g = np.load(os.path.join(GOLD, "dsterf_config2.npz"))
+    for kind in GENERATORS:
+        d, e = generate(kind, n, seed_for(2, n))
+        w, info = orc.dstebr(d, e)
+        assert info == 0 and np.all(np.diff(w) >= 0)
+        assert rel_err(w, g[f"{kind}_{n}"]) < 1e-13, (kind, n)
+
+
 @pytest.mark.parametrize("smlsiz", [0, 8, 32, 64])
 def test_dstebr_leaf_size_independent(orc, smlsiz):
     g = np.load(os.path.join(GOLD, "dsterf_eigs.npz"))
@@ -206,6 +218,80 @@ def test_query_and_argument_errors(orc):
     info, lw, liw = orc.query(25)
     assert (info, lw, liw) == (0, 1, 1)
     assert orc.query(-1)[0] == -1
+
+
+# BRDC_WPROD_NTHREADS (dstedc_dc_br.f:153-213), written out from the
+# reference's text: with no usable override the buffer count is a step of the
+# OpenMP team size NT (NT <= 1 -> 1, <= 8 -> 2, <= 28 -> 4, else 16, capped at
+# NT); a numeric OPENBLAS_BRDC_WPROD_THREADS gives max(1, min(value, NT));
+# blanks are skipped, any other character voids the override, and 0 counts as
+# no override.  Each case: (OMP_NUM_THREADS, override or None, expected).
+WPROD_CASES = [(1, None, 1), (2, None, 2), (8, None, 2), (9, None, 4), (28, None, 4),
+               (29, None, 16), (56, None, 16),
+               (8, "3", 3), (8, "99", 8), (8, "3x", 2), (8, " 5 ", 5), (8, "0", 2),
+               (1, "4", 1), (29, "7", 7), (56, "", 16)]
+
+_WPROD_PROBE = """
+import ctypes, os, sys
+lib = ctypes.CDLL(sys.argv[1], mode=os.RTLD_LAZY)
+f = getattr(lib, sys.argv[2])
+f.restype = ctypes.c_int
+print(f())
+"""
+
+
+def _wprod_in_subprocess(path, symbol, threads, override):
+    """OpenMP reads OMP_NUM_THREADS when its runtime starts, so each setting
+    runs in a process of its own."""
+    import subprocess
+    import sys
+    env = {k: v for k, v in os.environ.items() if k != "OPENBLAS_BRDC_WPROD_THREADS"}
+    env["OMP_NUM_THREADS"] = str(threads)
+    if override is not None:
+        env["OPENBLAS_BRDC_WPROD_THREADS"] = override
+    out = subprocess.run([sys.executable, "-c", _WPROD_PROBE, path, symbol], env=env,
+                         check=True, capture_output=True, text=True).stdout
+    return int(out.strip())
+
+
+@pytest.mark.parametrize("threads,override,expected", WPROD_CASES)
+def test_wprod_nthreads_table(orc, threads, override, expected):
+    assert _wprod_in_subprocess(orc.LIB_PATH, "brdc_oracle_wprod_nthreads", threads,
+                                override) == expected
+
+
+REF_DRIVER = os.path.join(os.path.dirname(GOLD), "..", "oracle", "_ref",
+                          "libref_dstedc_dc_br.so")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_DRIVER),
+                    reason="oracle/_ref not built (needs the reference tree and amdflang)")
+@pytest.mark.parametrize("threads,override,expected", WPROD_CASES)
+def test_wprod_nthreads_matches_reference_build(threads, override, expected):
+    """The reference's own BRDC_WPROD_NTHREADS, compiled from
+    cpu/src/dstedc_dc_br.f by oracle/Makefile's `ref` target, gives the same
+    table."""
+    assert _wprod_in_subprocess(REF_DRIVER, "brdc_wprod_nthreads_", threads,
+                                override) == expected
+
+
+def test_query_follows_wprod_threads(orc):
+    """The workspace query's LWORK = (16 + 2*NTHR)*N follows the buffer
+    count (dstedc_dc_br.f:51-59): at 29 threads NTHR is 16, so N = 4096 needs
+    48*4096 doubles; an override of 3 gives 22*4096."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, sys.argv[1]); import oracle; "
+            "print(*oracle.query(4096))")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for threads, override, nthr in ((29, None, 16), (8, "3", 3), (1, None, 1)):
+        env = {k: v for k, v in os.environ.items() if k != "OPENBLAS_BRDC_WPROD_THREADS"}
+        env["OMP_NUM_THREADS"] = str(threads)
+        if override is not None:
+            env["OPENBLAS_BRDC_WPROD_THREADS"] = override
+        out = subprocess.run([sys.executable, "-c", code, root], env=env, check=True,
+                             capture_output=True, text=True).stdout.split()
+        assert [int(x) for x in out] == [0, (16 + 2 * nthr) * 4096, 7 * 4096]
 
 
 def test_splits_zero_matrix_and_small(orc):
