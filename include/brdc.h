@@ -93,6 +93,23 @@ int brdc_dstebr_device_ex(int n, double *d_dev, const double *e_dev, void *works
                           const brdc_options *opts);
 
 /*
+ * Allocation-free device entry.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory or bring a workspace (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as brdc_dstebr_device_ex without
+ * the workspace, which comes from a static device array of
+ * BRDC_NOALLOC_BYTES per GPU (held by each GPU the library runs on until the
+ * process ends).  Orders up to BRDC_NOALLOC_MAX_N; a larger n returns
+ * BRDC_STATUS_INVALID_VALUE.  Calls no allocating routine.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two host threads) race and must be serialised by the caller.
+ */
+#define BRDC_NOALLOC_MAX_N 65536
+#define BRDC_NOALLOC_BYTES (20u << 20)
+int brdc_dstebr_device_noalloc(int n, double *d_dev, const double *e_dev, void *stream,
+                               int *info_dev, const brdc_options *opts);
+
+/*
  * Per-merge diagnostics of the last solve run in a workspace (the role of
  * TridiagMergeLevelDiagnostics, gpu/src/handle_impl.hpp:16-85, and of the
  * per-merge (N, K) record of the instrumented CPU copy).  Copies up to

@@ -23,10 +23,13 @@ EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
            "brdc_batch_plan_create", "brdc_batch_plan_destroy", "brdc_batch_workspace",
            "brdc_batch_upload", "brdc_batch_solve", "brdc_batch_plan_stats",
            "brdc_dstebr_grad_workspace", "brdc_dstebr_grad_device", "brdc_batch_grad_workspace",
-           "brdc_batch_grad")
+           "brdc_batch_grad", "brdc_dstebr_device_noalloc")
 
 KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
                   "finalize", "secular_stream", "secular_mid")
+# the allocation-free entry's static workspace (BRDC_NOALLOC_* in include/brdc.h)
+NOALLOC_MAX_N = 65536
+NOALLOC_BYTES = 20 << 20
 # merges of at least this many poles go to the streamed secular kernel
 # (kStreamMinK in csrc/brdc.hip)
 STREAM_MIN_K = 2048
@@ -69,6 +72,9 @@ def load() -> ctypes.CDLL:
     L.brdc_dstebr_device_ex.restype = ctypes.c_int
     L.brdc_dstebr_device_ex.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp,
                                         ctypes.POINTER(Options)]
+    L.brdc_dstebr_device_noalloc.restype = ctypes.c_int
+    L.brdc_dstebr_device_noalloc.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp,
+                                             ctypes.POINTER(Options)]
     L.brdc_merge_stats.restype = ctypes.c_int
     L.brdc_merge_stats.argtypes = [ctypes.c_int, _vp, ctypes.c_size_t, _vp,
                                    ctypes.POINTER(Options), _ip, ctypes.c_int]

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <climits>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1006,3 +1007,52 @@ int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const doub
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free device entry.  The workspace of brdc_dstebr_device lives in
+// a static device array, one copy per GPU, held until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: it is
+// used only to look up each device's address, and every launch gets that
+// address.  All calls on one GPU share its copy, so calls that overlap on a
+// GPU race; the kernels write every word they later read (counts, records,
+// INFO), so nothing needs clearing between calls.
+static __device__ __align__(256) unsigned char scratch[BRDC_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current device's scratch, looked up on the first call
+// on that device (with it current) and kept.
+int scratch_address(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return BRDC_STATUS_HIP_ERROR;
+    if (dev < 0 || dev >= kMaxDevices) return BRDC_STATUS_INTERNAL_ERROR;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return BRDC_STATUS_HIP_ERROR;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return BRDC_STATUS_SUCCESS;
+}
+}  // namespace
+
+extern "C" int brdc_dstebr_device_noalloc(int n, double* d_dev, const double* e_dev,
+                                          void* stream, int* info_dev,
+                                          const brdc_options* opts) {
+    if (n < 0 || n > BRDC_NOALLOC_MAX_N || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
+    if (n == 0) return BRDC_STATUS_SUCCESS;
+    if (!d_dev || (n > 1 && !e_dev)) return BRDC_STATUS_INVALID_POINTER;
+    const int leaf = leaf_of(opts);
+    const Layout L = tree_layout(n, leaf);
+    if (L.total > sizeof(scratch)) return BRDC_STATUS_INVALID_VALUE;
+    void* ws = nullptr;
+    const int st = scratch_address(&ws);
+    if (st != BRDC_STATUS_SUCCESS) return st;
+    return enqueue_solve(n, d_dev, e_dev, ws, L, static_cast<hipStream_t>(stream), info_dev,
+                         leaf);
+}

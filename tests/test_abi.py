@@ -162,3 +162,29 @@ def test_batch_entry_rejects_bad_arguments_without_launching():
     assert st == 0
     assert L.brdc_batch_solve(p, None, None, None, 0, None, None) == 0  # empty batch
     L.brdc_batch_plan_destroy(p)
+
+
+def test_noalloc_workspace_bound_covers_every_leaf():
+    """The allocation-free entry's static array holds the workspace of its
+    largest order at every leaf bound, and the Python mirror of its limits
+    matches the header."""
+    hdr = open(os.path.join(ROOT, "include", "brdc.h")).read()
+    assert int(re.search(r"#define BRDC_NOALLOC_MAX_N (\d+)", hdr).group(1)) == \
+        _lib.NOALLOC_MAX_N
+    assert re.search(r"#define BRDC_NOALLOC_BYTES \(20u << 20\)", hdr)
+    L = _lib.load()
+    for leaf in range(2, 65):
+        opts = _lib.options(leaf)
+        out = ctypes.c_size_t(0)
+        assert L.brdc_dstebr_device_workspace_ex(_lib.NOALLOC_MAX_N, ctypes.byref(opts),
+                                                 ctypes.byref(out)) == 0
+        assert out.value <= _lib.NOALLOC_BYTES, leaf
+
+
+def test_noalloc_argument_errors_before_any_device_work():
+    L = _lib.load()
+    assert L.brdc_dstebr_device_noalloc(-1, None, None, None, None, None) == 3
+    assert L.brdc_dstebr_device_noalloc(_lib.NOALLOC_MAX_N + 1, None, None, None, None,
+                                        None) == 3
+    assert L.brdc_dstebr_device_noalloc(0, None, None, None, None, None) == 0
+    assert L.brdc_dstebr_device_noalloc(10, None, None, None, None, None) == 2
