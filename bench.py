@@ -37,6 +37,12 @@ GENS = ("uniform", "normal", "toeplitz", "clustered")
 PEAK_FP64 = 78.6e12  # MI355X FP64 vector (= FP64 matrix) peak, FLOP/s
 
 
+def progress(msg):
+    """One line per finished section on stderr (the JSON line stays alone on
+    stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def rel_err(w, ref):
     return float(np.max(np.abs(w - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
@@ -579,13 +585,16 @@ def main():
     if not graphs:
         per_gen = per_gen_eager
 
+    progress(f"timed steps done: {value:.4f} ms/solve")
     prof = brdc.profile_kernels(solver, [(w_dev[g], d_dev[g], e_dev[g]) for g in GENS], stream)
+    progress("kernel-class profile done")
 
     # accuracy
     acc = {}
     cpu_rec, cpu_eigs, qp = (None, {}, None)
     if not args.no_cpu_baseline:
         cpu_rec, cpu_eigs = cpu_baseline(mats)
+        progress("cpu baseline done")
         qp = quad_probes(mats, outs)
     for g in GENS:
         d, e = mats[g]
@@ -666,13 +675,16 @@ def main():
     if cpu_rec:
         line["speedup_vs_cpu_baseline"] = round(cpu_rec["value"] / value, 2) if value > 0 else None
     if not args.no_batched:
+        progress("batched section starts")
         line["batched"] = bench_batched(max(3, args.steps // 2), max(1, args.warmup),
                                         want_cpu=not args.no_cpu_baseline,
                                         compare=not args.no_compare)
     if not args.no_grad:
+        progress("differentiated section starts")
         line["differentiated"] = bench_differentiated(max(3, args.steps // 2), max(1, args.warmup),
                                                       compare=not args.no_compare)
     if not args.no_vendor:
+        progress("vendor and config-3 sections start")
         line["vendor_baseline"] = bench_vendor()
         line["config3_and_host_entry"] = bench_config3_and_host_entry()
     print(json.dumps(line), flush=True)

@@ -32,8 +32,9 @@ typedef enum brdc_status {
 
 /* Tuning knobs of the device path; zero-initialise for the defaults. */
 typedef struct brdc_options {
-    int leaf_max;     /* largest leaf (rows) solved by the wavefront Jacobi
-                         kernel, 2..64; 0 selects the default (16) */
+    int leaf_max;     /* largest leaf (rows) of the tree, solved by the leaf
+                         kernel (implicit QL, one lane per leaf), 2..64;
+                         0 selects the default (16) */
     int reserved[7];  /* must be zero */
 } brdc_options;
 
@@ -62,6 +63,13 @@ int brdc_dstebr(int n, double *d, double *e, double *work, int lwork, int *iwork
  * convention of dstedc_dc_br_ (scripts/cpu/smoke_br.f90:24-27). */
 void brdc_dstebr_(const int *n, double *d, double *e, double *work, const int *lwork,
                   int *iwork, const int *liwork, int *info);
+
+/* The same entry under the reference's own symbol (SUBROUTINE DSTEDC_DC_BR,
+ * cpu/src/dstedc_dc_br.f:7-8, as gfortran and amdflang name it), so an
+ * unchanged Fortran caller links against this library in place of the
+ * reference's objects.  Weak: a strong definition in the program wins. */
+void dstedc_dc_br_(const int *n, double *d, double *e, double *work, const int *lwork,
+                   int *iwork, const int *liwork, int *info);
 
 /*
  * Device workspace query.  Replaces the reference's
@@ -113,12 +121,16 @@ int brdc_dstebr_device_noalloc(int n, double *d_dev, const double *e_dev, void *
  * Per-merge diagnostics of the last solve run in a workspace (the role of
  * TridiagMergeLevelDiagnostics, gpu/src/handle_impl.hpp:16-85, and of the
  * per-merge (N, K) record of the instrumented CPU copy).  Copies up to
- * max_merges records of 6 ints -- level (0 = root merge), first row, size,
- * K (non-deflated poles), G (close-pole Givens deflations), secular function
- * evaluations summed over the merge's roots -- into out_host, ordered by
- * level from the root down, and returns the number of merges (or a negative
- * brdc_status).  Synchronises the stream.
+ * max_merges records of BRDC_MERGE_STAT_FIELDS ints -- level (0 = root
+ * merge), first row, size, K (non-deflated poles), G (close-pole Givens
+ * deflations: close_deflated_total), secular function evaluations summed over
+ * the merge's roots (secular_iter_sum), T (tiny-z deflations:
+ * zero_deflated_total), the most evaluations of one root (secular_iter_max),
+ * roots that did not converge (secular_iter_failed) -- into out_host, ordered
+ * by level from the root down, and returns the number of merges (or a
+ * negative brdc_status).  Synchronises the stream.
  */
+#define BRDC_MERGE_STAT_FIELDS 9
 int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void *stream,
                      const brdc_options *opts, int *out_host, int max_merges);
 
@@ -137,6 +149,14 @@ int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void 
 #define BRDC_KERNEL_CLASSES 11
 int brdc_profile_begin(void);
 int brdc_profile_end(double *ms_out, int *launches_out, int nclasses);
+
+/* As brdc_profile_end, split by merge level (the per-level values_*_ms of
+ * TridiagMergeLevelDiagnostics): ms_out and launches_out hold (nlevels + 1)
+ * rows of nclasses entries; row 0 gathers the launches before the merges
+ * (norm, prep_e, leaf), row 1 + l the merges of level l (tree depth, 0 = the
+ * root merge; for a batch, forest height l, 1 = one level above the leaves).
+ * Levels beyond nlevels are dropped. */
+int brdc_profile_end_levels(double *ms_out, int *launches_out, int nclasses, int nlevels);
 
 /*
  * Batched solves: many independent tridiagonals of uneven sizes as one
@@ -177,6 +197,14 @@ int brdc_batch_upload(const brdc_batch_plan *plan, void *workspace, size_t works
  * within the matrix).  Asynchronous; returns a status for the launch. */
 int brdc_batch_solve(const brdc_batch_plan *plan, double *d_dev, const double *e_dev,
                      void *workspace, size_t workspace_bytes, void *stream, int *info_dev);
+
+/* Per-merge diagnostics of the last batched solve in a workspace, as
+ * brdc_merge_stats: records of BRDC_MERGE_STAT_FIELDS ints whose level is the
+ * forest height (1 = one level above the leaves) and whose first row counts in
+ * the concatenated rows; ordered by height, then by matrix.  Returns the
+ * number of merges (or a negative brdc_status); synchronises the stream. */
+int brdc_batch_merge_stats(const brdc_batch_plan *plan, const void *workspace,
+                           size_t workspace_bytes, void *stream, int *out_host, int max_merges);
 
 /* Plan summary: up to nout of (rows, matrices, leaves, forest heights, merges,
  * largest leaf, leaf bound).  Returns the number written or a negative status. */

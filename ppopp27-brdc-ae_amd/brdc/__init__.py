@@ -23,7 +23,7 @@ from . import _lib
 from ._lib import BrdcError, options
 
 __all__ = ["dstebr", "dstebr_device", "workspace_bytes", "merge_stats", "Solver", "BrdcError",
-           "profile_kernels", "GraphSolve", "BatchSolver", "options", "query"]
+           "profile_kernels", "profile_levels", "GraphSolve", "BatchSolver", "options", "query"]
 
 
 def query(n: int) -> tuple[int, int, int]:
@@ -99,22 +99,29 @@ class Solver:
         return d
 
     def merge_stats(self, stream=None):
-        """Per-merge (level, start, size, K, G, secular evaluations) of the last solve."""
+        """Per-merge records of the last solve: level, start, size, K, G
+        (Givens deflations), evals (secular evaluations, all roots), tiny
+        (tiny-z deflations), evals_max (most evaluations of one root),
+        failed (roots that did not converge)."""
         import torch
         L = _lib.load()
         st = stream if stream is not None else torch.cuda.current_stream()
         total = max(1, self.n)
-        buf = np.zeros(6 * total, dtype=np.int32)
+        buf = np.zeros(_lib.MERGE_STAT_FIELDS * total, dtype=np.int32)
         cnt = L.brdc_merge_stats(self.n, self.ws.data_ptr(), self.bytes, st.cuda_stream,
                                  ctypes.byref(self.opts), buf.ctypes.data_as(_lib._ip), total)
         if cnt < 0:
             _lib.check(-cnt, "merge stats")
-        rec = buf[: 6 * cnt].reshape(cnt, 6)
-        out = np.zeros(cnt, dtype=[("level", "i4"), ("start", "i4"), ("size", "i4"),
-                                   ("k", "i4"), ("givens", "i4"), ("evals", "i4")])
-        for i, name in enumerate(out.dtype.names):
-            out[name] = rec[:, i]
-        return out
+        return _merge_records(buf, cnt)
+
+
+def _merge_records(buf, cnt):
+    F = _lib.MERGE_STAT_FIELDS
+    rec = buf[: F * cnt].reshape(cnt, F)
+    out = np.zeros(cnt, dtype=[(name, "i4") for name in _lib.MERGE_STAT_NAMES])
+    for i, name in enumerate(_lib.MERGE_STAT_NAMES):
+        out[name] = rec[:, i]
+    return out
 
 
 class BatchSolver:
@@ -184,6 +191,21 @@ class BatchSolver:
                                                 self.info.data_ptr()),
                    "brdc_batch_solve")
         return d
+
+    def merge_stats(self, stream=None):
+        """Per-merge records of the last batched solve (fields as
+        Solver.merge_stats; level = forest height, start in the concatenated
+        rows)."""
+        import torch
+        L = _lib.load()
+        st = stream if stream is not None else torch.cuda.current_stream()
+        total = max(1, self.stats()["merges"])
+        buf = np.zeros(_lib.MERGE_STAT_FIELDS * total, dtype=np.int32)
+        cnt = L.brdc_batch_merge_stats(self._plan, self.ws.data_ptr(), self.bytes,
+                                       st.cuda_stream, buf.ctypes.data_as(_lib._ip), total)
+        if cnt < 0:
+            _lib.check(-cnt, "batch merge stats")
+        return _merge_records(buf, cnt)
 
     def split(self, w):
         """Per-matrix views of concatenated rows (numpy or torch)."""
@@ -264,6 +286,35 @@ def profile_kernels(solver: Solver, problems, stream=None):
             kern[name] = {"ms": float(tot_ms[i]), "launches": int(tot_n[i]),
                           "avg_us": float(tot_ms[i] * 1e3 / tot_n[i])}
     return {"kernels": kern, "flops": flops, "bytes": nbytes}
+
+
+def profile_levels(solver, w, d, e, stream=None, nlevels=64):
+    """Kernel-class milliseconds of one solve split by merge level (the
+    per-level timers of the reference's TridiagMergeLevelDiagnostics):
+    returns {level: {class: ms}} with level -1 for the launches before the
+    merges (norm, prep_e, leaf) and l >= 0 for the merges of tree depth l
+    (0 = root merge), or forest height l for a BatchSolver."""
+    import torch
+    L = _lib.load()
+    nc = len(_lib.KERNEL_CLASSES)
+    st = stream if stream is not None else torch.cuda.current_stream()
+    w.copy_(d)
+    torch.cuda.synchronize()
+    _lib.check(L.brdc_profile_begin(), "profile begin")
+    solver.solve(w, e, st)
+    ms = np.zeros((nlevels + 1) * nc)
+    cnt = np.zeros((nlevels + 1) * nc, dtype=np.int32)
+    _lib.check(L.brdc_profile_end_levels(ms.ctypes.data_as(_lib._dp),
+                                         cnt.ctypes.data_as(_lib._ip), nc, nlevels),
+               "profile end")
+    ms = ms.reshape(nlevels + 1, nc)
+    cnt = cnt.reshape(nlevels + 1, nc)
+    out = {}
+    for row in range(nlevels + 1):
+        if cnt[row].sum():
+            out[row - 1] = {name: float(ms[row, i]) for i, name in
+                            enumerate(_lib.KERNEL_CLASSES) if cnt[row, i]}
+    return out
 
 
 class GraphSolve:

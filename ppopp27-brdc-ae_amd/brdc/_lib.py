@@ -17,16 +17,21 @@ STATUS = {0: "success", 2: "invalid pointer", 3: "invalid value", 4: "invalid wo
           5: "alloc failed", 6: "hip error", 7: "internal error"}
 
 #: every symbol include/brdc.h declares
-EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "brdc_dstebr_device_workspace",
+EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "dstedc_dc_br_", "brdc_dstebr_device_workspace",
            "brdc_dstebr_device_workspace_ex", "brdc_dstebr_device", "brdc_dstebr_device_ex",
            "brdc_merge_stats", "brdc_status_string", "brdc_profile_begin", "brdc_profile_end",
            "brdc_batch_plan_create", "brdc_batch_plan_destroy", "brdc_batch_workspace",
            "brdc_batch_upload", "brdc_batch_solve", "brdc_batch_plan_stats",
            "brdc_dstebr_grad_workspace", "brdc_dstebr_grad_device", "brdc_batch_grad_workspace",
-           "brdc_batch_grad", "brdc_dstebr_device_noalloc")
+           "brdc_batch_grad", "brdc_dstebr_device_noalloc", "brdc_profile_end_levels",
+           "brdc_batch_merge_stats")
 
 KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
                   "finalize", "secular_stream", "secular_mid")
+# ints per merge record of brdc_merge_stats / brdc_batch_merge_stats
+MERGE_STAT_FIELDS = 9
+MERGE_STAT_NAMES = ("level", "start", "size", "k", "givens", "evals", "tiny", "evals_max",
+                    "failed")
 # the allocation-free entry's static workspace (BRDC_NOALLOC_* in include/brdc.h)
 NOALLOC_MAX_N = 65536
 NOALLOC_BYTES = 20 << 20
@@ -82,6 +87,10 @@ def load() -> ctypes.CDLL:
     L.brdc_profile_begin.argtypes = []
     L.brdc_profile_end.restype = ctypes.c_int
     L.brdc_profile_end.argtypes = [_dp, _ip, ctypes.c_int]
+    L.brdc_profile_end_levels.restype = ctypes.c_int
+    L.brdc_profile_end_levels.argtypes = [_dp, _ip, ctypes.c_int, ctypes.c_int]
+    L.brdc_batch_merge_stats.restype = ctypes.c_int
+    L.brdc_batch_merge_stats.argtypes = [_vp, _vp, ctypes.c_size_t, _vp, _ip, ctypes.c_int]
     L.brdc_status_string.restype = ctypes.c_char_p
     L.brdc_status_string.argtypes = [ctypes.c_int]
     L.brdc_batch_plan_create.restype = ctypes.c_int

@@ -9,17 +9,27 @@
 Columns: mode, n, prep_ms (input preparation), cpu_ms (LAPACK DSTERF on the
 host, the reference's acceptance oracle; the reference puts its CPU solver
 there), self_ms (this GPU solve, HIP-event timed, median of --iters after
---warmup), self_err (max |w - w_cpu| / max |w_cpu|), rslv_ms (rocSOLVER
-STEDC values-only on the same GPU, one run: the vendor column the reference
-fills with cuSOLVER), rslv/self and rslv_err.  --run-rocsolver replaces
---run-cusolver (also accepted).  Dense inputs (--input without
---input-tridiag) need the two-stage reduction, which this path does not
-include.
+--warmup), self_err (the reference's error column: the 2-norm relative
+difference ||w - w_cpu|| / ||w_cpu||, tridiag_dc_bench.cu:747-757),
+self_maxerr (max |w - w_cpu| / max |w_cpu|, the acceptance measure of
+scripts/cpu/smoke_br.f90:32-33), rslv_ms (rocSOLVER STEDC values-only on the
+same GPU, one run: the vendor column the reference fills with cuSOLVER),
+rslv/self and rslv_err (2-norm).  --run-rocsolver replaces --run-cusolver
+(also accepted).  Dense inputs (--input without --input-tridiag) need the
+two-stage reduction, which this path does not include.
 
-Data modes: normal, uniform, toeplitz, clustered (the generators of
-brdc.generators) and bench (D_i = n + i/4 + U(-1/2, 1/2), E_i = U(0.05, 0.95):
-the distribution of the reference's bench mode, drawn from this package's
-splitmix64 stream rather than std::mt19937).
+Data modes (--data, comma-separated, default bench):
+* the reference bench's own inputs, bit for bit (brdc.paper_data):
+  bench, test, normal, uniform, toeplitz, clustered;
+* this package's splitmix64 generators (brdc.generators, seed
+  seed_for(2, n)): splitmix-uniform, splitmix-normal, splitmix-toeplitz,
+  splitmix-clustered.
+
+--tridiag-diagnostics=1 prints, per merge level, the fields of the
+reference's TridiagMergeLevelDiagnostics that this path has: merges, rows,
+node sizes, non-deflated poles, zero (tiny-z) and close (Givens) deflations,
+secular evaluations (sum, most for one root, failed roots) and the level's
+milliseconds by kernel class.
 """
 from __future__ import annotations
 
@@ -28,7 +38,12 @@ import time
 
 import numpy as np
 
-from .generators import GENERATORS, SplitMix64, generate, read_de_bin, seed_for, write_de_bin
+from . import paper_data
+from .generators import GENERATORS, generate, read_de_bin, seed_for, write_de_bin
+
+SPLITMIX_PREFIX = "splitmix-"
+MODES = ("bench", "test") + paper_data.PAPER_MODES + tuple(SPLITMIX_PREFIX + g
+                                                          for g in GENERATORS)
 
 
 def _flag(argv, name, default=None):
@@ -44,19 +59,29 @@ def _int_flag(argv, name, default):
     return default if v is None else int(v)
 
 
-def bench_data(n: int, seed: int):
-    g = SplitMix64(seed)
-    d = float(n) + 0.25 * np.arange(n) + (g.uniform(n) - 0.5)
-    e = 0.05 + 0.9 * g.uniform(max(n - 1, 0))
-    return d, e
-
-
-def make_input(mode: str, n: int):
+def make_input(mode: str, n: int, size_index: int = 0):
+    """(D, E) of one --data mode at order n; size_index is n's position in
+    --sizes (the bench mode's seed, tridiag_dc_bench.cu:1513)."""
     if mode == "bench":
-        return bench_data(n, seed_for(0, n))
-    if mode in GENERATORS:
-        return generate(mode, n, seed_for(2, n))
-    raise SystemExit(f"unknown --data mode {mode!r} (bench, {', '.join(GENERATORS)})")
+        return paper_data.bench_data(n, size_index)
+    if mode == "test":
+        return paper_data.test_data(n)
+    if mode in paper_data.PAPER_MODES:
+        return paper_data.paper_data(mode, n)
+    if mode.startswith(SPLITMIX_PREFIX) and mode[len(SPLITMIX_PREFIX):] in GENERATORS:
+        return generate(mode[len(SPLITMIX_PREFIX):], n, seed_for(2, n))
+    raise SystemExit(f"unknown --data mode {mode!r} (expected one of {', '.join(MODES)})")
+
+
+def rel_err_2norm(w, ref):
+    """||w - ref|| / ||ref|| (tridiag_dc_bench.cu:747-757)."""
+    num = float(np.sum((w - ref) ** 2))
+    den = float(np.sum(ref * ref))
+    return float(np.sqrt(num / max(den, 1.0e-30)))
+
+
+def rel_err_max(w, ref):
+    return float(np.max(np.abs(w - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
 
 def _median_gpu_ms(fn, warmup, iters):
@@ -106,10 +131,12 @@ def run_case(mode, d, e, prep_ms, opts, out=sys.stdout):
         w_self = w.cpu().numpy()
         row["self_info"] = int(s.info.item())
         if w_cpu is not None:
-            row["self_err"] = float(np.max(np.abs(w_self - w_cpu)) /
-                                    max(np.max(np.abs(w_cpu)), 1e-300))
+            row["self_err"] = rel_err_2norm(w_self, w_cpu)
+            row["self_maxerr"] = rel_err_max(w_self, w_cpu)
         if opts["diagnostics"]:
+            from . import profile_levels
             row["merges"] = s.merge_stats()
+            row["level_ms"] = profile_levels(s, w, dt, et)
     if opts["run_rocsolver"]:
         import torch
         from .comparators import rocsolver_eigvals
@@ -124,16 +151,16 @@ def run_case(mode, d, e, prep_ms, opts, out=sys.stdout):
         wr = wr.cpu().numpy()
         ref = w_cpu if w_cpu is not None else w_self
         if ref is not None:
-            row["rslv_err"] = float(np.max(np.abs(wr - ref)) / max(np.max(np.abs(ref)), 1e-300))
+            row["rslv_err"] = rel_err_2norm(wr, ref)
         if "self_ms" in row:
             row["rslv/self"] = row["rslv_ms"] / row["self_ms"]
     return row
 
 
-COLUMNS = ("mode", "n", "prep_ms", "cpu_ms", "self_ms", "self_err", "rslv_ms", "rslv/self",
-           "rslv_err")
-WIDTH = {"mode": 12, "n": 8, "prep_ms": 12, "cpu_ms": 14, "self_ms": 14, "self_err": 16,
-         "rslv_ms": 14, "rslv/self": 12, "rslv_err": 16}
+COLUMNS = ("mode", "n", "prep_ms", "cpu_ms", "self_ms", "self_err", "self_maxerr", "rslv_ms",
+           "rslv/self", "rslv_err")
+WIDTH = {"mode": 20, "n": 8, "prep_ms": 12, "cpu_ms": 14, "self_ms": 14, "self_err": 16,
+         "self_maxerr": 16, "rslv_ms": 14, "rslv/self": 12, "rslv_err": 16}
 
 
 def format_row(row, cols):
@@ -146,14 +173,28 @@ def format_row(row, cols):
     return "".join(out)
 
 
-def format_merges(stats):
-    lines = ["  level  merges      rows     K_sum   K_max  givens  evals/root"]
+def format_merges(stats, level_ms=None):
+    """Per-level table of the merge diagnostics (and kernel milliseconds)."""
+    classes = ("prepare", "deflate", "secular", "secular_mid", "secular_stream", "wprod",
+               "rows", "finalize")
+    head = ("  level  merges      rows  size_min  size_max     K_sum  zero_defl  close_defl"
+            "  iter_sum  iter_max  failed")
+    if level_ms is not None:
+        head += "  ms(" + " ".join(classes) + ")"
+    lines = [head]
     for lev in sorted(set(int(x) for x in stats["level"])):
         m = stats[stats["level"] == lev]
-        ks = int(m["k"].sum())
-        lines.append(f"  {lev:5d} {len(m):7d} {int(m['size'].sum()):9d} {ks:9d} "
-                     f"{int(m['k'].max()):7d} {int(m['givens'].sum()):7d} "
-                     f"{(m['evals'].sum() / max(ks, 1)):11.2f}")
+        line = (f"  {lev:5d} {len(m):7d} {int(m['size'].sum()):9d} {int(m['size'].min()):9d} "
+                f"{int(m['size'].max()):9d} {int(m['k'].sum()):9d} {int(m['tiny'].sum()):10d} "
+                f"{int(m['givens'].sum()):11d} {int(m['evals'].sum()):9d} "
+                f"{int(m['evals_max'].max()):9d} {int(m['failed'].sum()):7d}")
+        if level_ms is not None:
+            per = level_ms.get(lev, {})
+            line += "  " + " ".join(f"{per.get(c, 0.0):.4f}" for c in classes)
+        lines.append(line)
+    if level_ms is not None and -1 in level_ms:
+        lines.append("  before the merges (ms): " + " ".join(
+            f"{k}={v:.4f}" for k, v in level_ms[-1].items()))
     return "\n".join(lines)
 
 
@@ -180,7 +221,7 @@ def main(argv=None, out=sys.stdout):
     cols = [c for c in COLUMNS if
             (c not in ("cpu_ms",) or opts["run_cpu"]) and
             (c not in ("self_ms",) or opts["run_self"]) and
-            (c != "self_err" or (opts["run_cpu"] and opts["run_self"])) and
+            (c not in ("self_err", "self_maxerr") or (opts["run_cpu"] and opts["run_self"])) and
             (c not in ("rslv_ms",) or opts["run_rocsolver"]) and
             (c != "rslv/self" or (opts["run_self"] and opts["run_rocsolver"])) and
             (c != "rslv_err" or (opts["run_rocsolver"] and (opts["run_cpu"] or opts["run_self"])))]
@@ -197,9 +238,9 @@ def main(argv=None, out=sys.stdout):
         cases.append(("tridiag_bin", d, e, (time.perf_counter() - t0) * 1e3))
     else:
         for mode in modes:
-            for n in sizes:
+            for idx, n in enumerate(sizes):
                 t0 = time.perf_counter()
-                d, e = make_input(mode, n)
+                d, e = make_input(mode, n, idx)
                 cases.append((mode, d, e, (time.perf_counter() - t0) * 1e3))
     for mode, d, e, prep in cases:
         if output_path:
@@ -212,7 +253,7 @@ def main(argv=None, out=sys.stdout):
         rows.append(row)
         print(format_row(row, cols), file=out, flush=True)
         if opts["diagnostics"] and "merges" in row:
-            print(format_merges(row["merges"]), file=out, flush=True)
+            print(format_merges(row["merges"], row.get("level_ms")), file=out, flush=True)
     return rows
 
 

@@ -62,6 +62,7 @@ int stream_groups() {
     return cached[dev];
 }
 constexpr size_t kAlign = 256;
+constexpr int kMergeStatFields = BRDC_MERGE_STAT_FIELDS;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -176,9 +177,16 @@ T* at(void* ws, size_t off) {
 // calling host thread, every launch is bracketed by two events on its stream.
 enum KernelClass { kNorm, kPrepE, kLeaf, kPrepare, kDeflate, kSecular, kWprod, kRows, kFinalize,
                    kSecularStream, kSecularMid, kNumClasses };
+// Each mark also carries the merge level it ran for (tree depth, 0 = root
+// merge, or forest height for a batch; -1 before the merges).
+struct ProfMark {
+    int cls, level;
+    hipEvent_t a, b;
+};
 struct Profiler {
     bool on = false;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    int level = -1;  // level of the launches being enqueued
+    std::vector<ProfMark> marks;
 };
 thread_local Profiler g_prof;
 
@@ -194,7 +202,7 @@ struct ProfScope {
     ~ProfScope() {
         if (!g_prof.on || !a || !b) return;
         (void)hipEventRecord(b, st);
-        g_prof.marks.push_back({cls, {a, b}});
+        g_prof.marks.push_back({cls, g_prof.level, a, b});
     }
 };
 
@@ -229,6 +237,7 @@ struct LevelRun {
     int maxsz;
     size_t counts_rec;  // first merge record of the level
     bool any_nonroot;   // some merge keeps boundary rows (wprod/rows run)
+    int report;         // level in diagnostics: tree depth, or forest height
 };
 
 // A whole solve: rows, matrices, leaves and levels (deepest first).
@@ -254,6 +263,7 @@ Schedule tree_schedule(int n, int leaf) {
         r.maxsz = max_node_size(n, depth);
         r.counts_rec = (size_t)r.merges - 1;
         r.any_nonroot = (depth > 0);
+        r.report = depth;
         S.levels.push_back(r);
     }
     return S;
@@ -326,7 +336,11 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
     int* rend = at<int>(ws, L.rend);
     int* fl = at<int>(ws, L.fl);
 
+    struct LevelReset {
+        ~LevelReset() { g_prof.level = -1; }
+    } level_reset;
     for (const LevelRun& lv : S.levels) {
+        g_prof.level = lv.report;
         const Geo g = lv.geo;
         const int merges = lv.merges;
         const int maxsz = lv.maxsz;
@@ -595,6 +609,7 @@ Schedule batch_schedule(const brdc_batch_plan& p, void* ws) {
         r.maxsz = h.maxsz;
         r.counts_rec = h.counts_rec;
         r.any_nonroot = h.any_nonroot;
+        r.report = (int)S.levels.size() + 1;
         S.levels.push_back(r);
     }
     return S;
@@ -675,13 +690,16 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
             int s, sz;
             node_geom(n, depth, i, &s, &sz);
             const int* c = &counts[(size_t)kCountStride * ((1 << depth) - 1 + i)];
-            int* o = out_host + 6 * rec;
+            int* o = out_host + kMergeStatFields * rec;
             o[0] = depth;
             o[1] = s;
             o[2] = sz;
             o[3] = c[0];
             o[4] = c[1];
             o[5] = c[3];
+            o[6] = c[2];
+            o[7] = c[5];
+            o[8] = c[6];
             rec++;
         }
     }
@@ -690,33 +708,60 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
 
 int brdc_profile_begin(void) {
     for (auto& m : g_prof.marks) {
-        (void)hipEventDestroy(m.second.first);
-        (void)hipEventDestroy(m.second.second);
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
     }
     g_prof.marks.clear();
     g_prof.on = true;
     return BRDC_STATUS_SUCCESS;
 }
 
-int brdc_profile_end(double* ms_out, int* launches_out, int nclasses) {
+int brdc_profile_end_levels(double* ms_out, int* launches_out, int nclasses, int nlevels) {
     g_prof.on = false;
     int status = BRDC_STATUS_SUCCESS;
-    std::vector<double> ms(kNumClasses, 0.0);
-    std::vector<int> cnt(kNumClasses, 0);
+    if (nclasses < 0 || nlevels < 0) status = BRDC_STATUS_INVALID_VALUE;
+    const int nc = std::min(std::max(nclasses, 0), (int)kNumClasses);
+    const size_t rows = (size_t)std::max(nlevels, 0) + 1;
+    if (ms_out)
+        for (size_t x = 0; x < rows * (size_t)std::max(nclasses, 0); x++) ms_out[x] = 0.0;
+    if (launches_out)
+        for (size_t x = 0; x < rows * (size_t)std::max(nclasses, 0); x++) launches_out[x] = 0;
     for (auto& m : g_prof.marks) {
         float t = 0.0f;
-        if (hipEventSynchronize(m.second.second) != hipSuccess ||
-            hipEventElapsedTime(&t, m.second.first, m.second.second) != hipSuccess)
+        if (hipEventSynchronize(m.b) != hipSuccess || hipEventElapsedTime(&t, m.a, m.b) != hipSuccess)
             status = BRDC_STATUS_HIP_ERROR;
-        ms[m.first] += t;
-        cnt[m.first] += 1;
-        (void)hipEventDestroy(m.second.first);
-        (void)hipEventDestroy(m.second.second);
+        // row 0: launches before the merges; row 1 + l: merge level l
+        const long row = (long)m.level + 1;
+        if (m.cls < nc && row >= 0 && row < (long)rows) {
+            const size_t at = (size_t)row * (size_t)nclasses + (size_t)m.cls;
+            if (ms_out) ms_out[at] += t;
+            if (launches_out) launches_out[at] += 1;
+        }
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
     }
     g_prof.marks.clear();
-    for (int c = 0; c < nclasses && c < kNumClasses; c++) {
-        if (ms_out) ms_out[c] = ms[c];
-        if (launches_out) launches_out[c] = cnt[c];
+    return status;
+}
+
+int brdc_profile_end(double* ms_out, int* launches_out, int nclasses) {
+    // every level folded into one row per class
+    const int nc = std::max(nclasses, 0);
+    int top = -1;
+    for (const auto& m : g_prof.marks) top = std::max(top, m.level);
+    const size_t rows = (size_t)(top + 2);
+    std::vector<double> ms(rows * (size_t)nc, 0.0);
+    std::vector<int> cnt(rows * (size_t)nc, 0);
+    const int status = brdc_profile_end_levels(ms.data(), cnt.data(), nclasses, top + 1);
+    for (int c = 0; c < nc; c++) {
+        double t = 0.0;
+        int k = 0;
+        for (size_t r = 0; r < rows; r++) {
+            t += ms[r * (size_t)nc + (size_t)c];
+            k += cnt[r * (size_t)nc + (size_t)c];
+        }
+        if (ms_out) ms_out[c] = t;
+        if (launches_out) launches_out[c] = k;
     }
     return status;
 }
@@ -773,6 +818,14 @@ int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork
 void brdc_dstebr_(const int* n, double* d, double* e, double* work, const int* lwork, int* iwork,
                   const int* liwork, int* info) {
     *info = brdc_dstebr(*n, d, e, work, *lwork, iwork, *liwork);
+}
+
+// The reference's own Fortran symbol, weak so that a strong definition linked
+// into the same program (the reference's objects) takes precedence.
+__attribute__((weak)) void dstedc_dc_br_(const int* n, double* d, double* e, double* work,
+                                         const int* lwork, int* iwork, const int* liwork,
+                                         int* info) {
+    brdc_dstebr_(n, d, e, work, lwork, iwork, liwork, info);
 }
 
 int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
@@ -894,6 +947,42 @@ int brdc_batch_workspace(const brdc_batch_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return BRDC_STATUS_INVALID_POINTER;
     *bytes = plan->layout.total;
     return BRDC_STATUS_SUCCESS;
+}
+
+int brdc_batch_merge_stats(const brdc_batch_plan* plan, const void* workspace,
+                           size_t workspace_bytes, void* stream, int* out_host, int max_merges) {
+    if (!plan || !workspace || (!out_host && max_merges > 0)) return -BRDC_STATUS_INVALID_POINTER;
+    const Layout& L = plan->layout;
+    if (workspace_bytes < L.total) return -BRDC_STATUS_INVALID_WORKSPACE;
+    const int total = (int)plan->merge_records;
+    if (total == 0) return 0;
+    std::vector<int> counts((size_t)kCountStride * total);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(counts.data(), static_cast<const char*>(workspace) + L.counts,
+                       counts.size() * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -BRDC_STATUS_HIP_ERROR;
+    int rec = 0;
+    for (size_t h = 0; h < plan->levels.size(); h++) {
+        const HostLevel& lv = plan->levels[h];
+        for (int m = 0; m < lv.merges; m++) {
+            if (rec >= max_merges) return total;
+            const int4 it = plan->items[lv.plan_at + (size_t)m];
+            const int* c = &counts[(size_t)kCountStride * (lv.counts_rec + (size_t)m)];
+            int* o = out_host + kMergeStatFields * rec;
+            o[0] = (int)h + 1;
+            o[1] = it.x;
+            o[2] = it.y;
+            o[3] = c[0];
+            o[4] = c[1];
+            o[5] = c[3];
+            o[6] = c[2];
+            o[7] = c[5];
+            o[8] = c[6];
+            rec++;
+        }
+    }
+    return total;
 }
 
 int brdc_batch_plan_stats(const brdc_batch_plan* plan, int* out, int nout) {

@@ -379,6 +379,8 @@ __global__ __launch_bounds__(kWideThreads) void k_wide_scatter(
         counts[kCountStride * merge + 2] = Tn;
         counts[kCountStride * merge + 3] = 0;
         counts[kCountStride * merge + 4] = 0;
+        counts[kCountStride * merge + 5] = 0;
+        counts[kCountStride * merge + 6] = 0;
     }
 }
 

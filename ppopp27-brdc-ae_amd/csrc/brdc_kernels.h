@@ -13,7 +13,8 @@
 //     w    product-form update vector                               [wprod]
 //     qo*  new boundary rows of the roots                           [rows]
 //   counts: kCountStride ints per merge (K, G, T, secular evaluations,
-//           roots claimed by the streamed secular kernel; the rest spare)
+//           roots claimed by the streamed secular kernel, the most
+//           evaluations of one root, roots that did not converge; one spare)
 // Every kernel derives its merge geometry from (n, depth, merge index), so a
 // level is a fixed launch sequence with no host round trip.
 #pragma once
@@ -838,6 +839,8 @@ __global__ __launch_bounds__(T) void k_deflate(
         counts[kCountStride * merge + 2] = Tn;
         counts[kCountStride * merge + 3] = 0;
         counts[kCountStride * merge + 4] = 0;
+        counts[kCountStride * merge + 5] = 0;
+        counts[kCountStride * merge + 6] = 0;
     }
 }
 
@@ -1007,15 +1010,16 @@ __device__ __forceinline__ void secular_side(const double2* __restrict__ dz, int
 }
 
 // One group of 64/L roots of one merge; dz is the merge's pole list (d_j,
-// z_j^2), in global memory or staged in LDS.  Returns the group's secular
-// iteration count (summed over the wave).
+// z_j^2), in global memory or staged in LDS.  Returns the lane's root's
+// secular evaluation count (on the root's first lane; 0 elsewhere) and adds a
+// root that did not converge to *failed.
 __device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, int K, int L,
                                              int grp, double rho, const double* __restrict__ ad,
                                              const double* __restrict__ az,
                                              const double2* __restrict__ dz,
                                              double* __restrict__ rlam, double2* __restrict__ rs,
                                              double* __restrict__ rn1,
-                                             double* __restrict__ rn2) {
+                                             double* __restrict__ rn2, int* failed) {
     const int lane = threadIdx.x & (kWave - 1);
     const int i = grp * (kWave / L) + lane / L;  // this lane's root
     const int c = lane & (L - 1);
@@ -1067,9 +1071,28 @@ __device__ __forceinline__ int secular_group(const FailSite& fs, int s, int sz, 
         rn1[s + i] = r.near1;
         rn2[s + i] = r.near2;
         iters = r.iters;
-        if (r.info) report_item_failure(fs, sz);
+        if (r.info) {
+            report_item_failure(fs, sz);
+            *failed += 1;
+        }
     }
     return iters;
+}
+
+// Adds a wave's evaluation count, its most evaluations of one root and its
+// non-converged roots to a merge's counters (one atomic of each per wave).
+__device__ __forceinline__ void add_secular_counts(int* __restrict__ rec, int iters, int imax,
+                                                   int failed) {
+    for (int o = 32; o > 0; o >>= 1) {
+        iters += __shfl_down(iters, o, kWave);
+        imax = max(imax, __shfl_down(imax, o, kWave));
+        failed += __shfl_down(failed, o, kWave);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (iters) atomicAdd(&rec[3], iters);
+        if (imax) atomicMax(&rec[5], imax);
+        if (failed) atomicAdd(&rec[6], failed);
+    }
 }
 
 // All roots of the level's merges with K < kstream.  Work is split into
@@ -1109,21 +1132,26 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
         const int s = it.s, sz = it.sz;
         const FailSite fs = fail_site(info, moff, g.n, it);
         const double rho = fabs(2.0 * ew[s + sz / 2 - 1]);
-        int iters = 0;
+        int iters = 0, imax = 0, failed = 0;
         if (K <= stage_k) {
             for (int j = lane; j < K; j += kWave) stage[j] = adz[s + j];
             __builtin_amdgcn_wave_barrier();
-            for (int grp = slot; grp * rpw < K; grp += slots)
-                iters += secular_group(fs, s, sz, K, L, grp, rho, ad, az, stage, rlam, rs, rn1,
-                                       rn2);
+            for (int grp = slot; grp * rpw < K; grp += slots) {
+                const int it = secular_group(fs, s, sz, K, L, grp, rho, ad, az, stage, rlam, rs,
+                                             rn1, rn2, &failed);
+                iters += it;
+                imax = max(imax, it);
+            }
             __builtin_amdgcn_wave_barrier();
         } else {
-            for (int grp = slot; grp * rpw < K; grp += slots)
-                iters += secular_group(fs, s, sz, K, L, grp, rho, ad, az, adz + s, rlam, rs, rn1,
-                                       rn2);
+            for (int grp = slot; grp * rpw < K; grp += slots) {
+                const int it = secular_group(fs, s, sz, K, L, grp, rho, ad, az, adz + s, rlam,
+                                             rs, rn1, rn2, &failed);
+                iters += it;
+                imax = max(imax, it);
+            }
         }
-        for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
-        if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
+        add_secular_counts(counts_rw + kCountStride * merge, iters, imax, failed);
     }
 }
 
@@ -1257,7 +1285,10 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
                         rn1[s + i] = v0;
                         rn2[s + i] = v1;
                     }
-                    if (i == 0) atomicAdd(&counts_rw[kCountStride * m + 3], 1);
+                    if (i == 0) {
+                        atomicAdd(&counts_rw[kCountStride * m + 3], 1);
+                        atomicMax(&counts_rw[kCountStride * m + 5], 1);
+                    }
                 }
             }
         }
@@ -1288,8 +1319,14 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
                 rs[s + i] = make_double2(dz[r.org].x, r.tau);
                 rn1[s + i] = r.near1;
                 rn2[s + i] = r.near2;
-                if (r.iters) atomicAdd(&counts_rw[kCountStride * m + 3], r.iters);
-                if (r.info) report_item_failure(fail_site(info, moff, g.n, g.item(m)), sz);
+                if (r.iters) {
+                    atomicAdd(&counts_rw[kCountStride * m + 3], r.iters);
+                    atomicMax(&counts_rw[kCountStride * m + 5], r.iters);
+                }
+                if (r.info) {
+                    report_item_failure(fail_site(info, moff, g.n, g.item(m)), sz);
+                    atomicAdd(&counts_rw[kCountStride * m + 6], 1);
+                }
             }
         }
     }
@@ -1744,7 +1781,7 @@ void k_secular_stream(
         SecularSolve<PoleView>& st = roots[owner ? wave : 0][lane];
         int i = -1;              // this lane's root in the slot, -1 when it needs one
         bool exhausted = false;  // the merge's counter has run out
-        int iters = 0;
+        int iters = 0, imax = 0, failed = 0;
         int parity = 0;
         for (;;) {
             if (owner) {
@@ -1849,18 +1886,19 @@ void k_secular_stream(
                         rs[s + i] = make_double2(dz[r.org].x, r.tau);
                         rn1[s + i] = r.near1;
                         rn2[s + i] = r.near2;
-                        if (r.info) report_item_failure(fs, sz);
+                        if (r.info) {
+                            report_item_failure(fs, sz);
+                            failed += 1;
+                        }
                         iters += r.iters;
+                        imax = max(imax, r.iters);
                     }
                     i = -1;
                 }
             }
             parity ^= 1;
         }
-        if (owner) {
-            for (int o = 32; o > 0; o >>= 1) iters += __shfl_down(iters, o, kWave);
-            if (lane == 0 && iters) atomicAdd(&counts_rw[kCountStride * merge + 3], iters);
-        }
+        if (owner) add_secular_counts(counts_rw + kCountStride * merge, iters, imax, failed);
         __syncthreads();  // pass_*[] of this merge are read before the next merge rewrites them
     }
 }

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "brdc.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(brdc_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(brdc_\w+|dstedc_dc_br_)\s*\(", txt)))
 
 
 def test_library_exports_every_header_symbol():
@@ -188,3 +188,43 @@ def test_noalloc_argument_errors_before_any_device_work():
                                         None) == 3
     assert L.brdc_dstebr_device_noalloc(0, None, None, None, None, None) == 0
     assert L.brdc_dstebr_device_noalloc(10, None, None, None, None, None) == 2
+
+
+AMDFLANG = "/opt/rocm/llvm/bin/amdflang"
+
+
+@pytest.mark.skipif(not os.path.exists(AMDFLANG), reason="amdflang not available")
+@pytest.mark.parametrize("route", ["reference_symbol", "renamed_reference"])
+def test_fortran_caller_links_and_runs(tmp_path, route):
+    """An unchanged Fortran caller of DSTEDC_DC_BR (tests/cpu_harness/
+    fortran_caller.f90, LP64, arguments by reference like
+    scripts/cpu/smoke_br.f90:24-27) links against libbrdc.so in place of the
+    reference's objects -- through the library's dstedc_dc_br_ symbol, or
+    with its reference renamed to brdc_dstebr_ by objcopy -- and the
+    workspace queries and argument errors behave as dstedc_dc_br.f:44-75
+    defines them."""
+    import shutil
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpu_harness", "fortran_caller.f90")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    exe = str(tmp_path / "caller")
+    if route == "reference_symbol":
+        subprocess.run([AMDFLANG, "-O1", src, "-o", exe, f"-L{libdir}", "-lbrdc",
+                        f"-Wl,-rpath,{libdir}"], check=True)
+    else:
+        if shutil.which("objcopy") is None:
+            pytest.skip("objcopy not available")
+        obj = str(tmp_path / "caller.o")
+        subprocess.run([AMDFLANG, "-O1", "-c", src, "-o", obj], check=True)
+        subprocess.run(["objcopy", "--redefine-sym", "dstedc_dc_br_=brdc_dstebr_", obj],
+                       check=True)
+        subprocess.run([AMDFLANG, obj, "-o", exe, f"-L{libdir}", "-lbrdc",
+                        f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    got = {ln.split()[0]: ln.split()[1:] for ln in out.splitlines() if ln.strip()}
+    assert got["query4096"] == ["0", str(18 * 4096), str(7 * 4096)]
+    assert got["query25"] == ["0", "1", "1"]
+    assert got["badn"] == ["-1"]
+    assert got["smalllwork"] == ["-5"]
+    assert got["smallliwork"] == ["-7"]
+    assert got["n1"] == ["0", "3.500"]

@@ -258,9 +258,22 @@ def test_harness_gpu_columns(gpu):
     import io
     from brdc import harness
     out = io.StringIO()
-    rows = harness.main(["--sizes=300", "--data=normal,toeplitz", "--iters=3", "--warmup=1",
-                         "--run-rocsolver=1", "--tridiag-diagnostics=1"], out=out)
+    rows = harness.main(["--sizes=300,5000", "--data=normal,toeplitz,clustered,splitmix-normal",
+                         "--iters=3", "--warmup=1", "--run-rocsolver=1",
+                         "--tridiag-diagnostics=1"], out=out)
+    assert len(rows) == 8
     for r in rows:
-        assert r["self_info"] == 0 and r["self_err"] <= TOL and r["rslv_err"] <= TOL
-        assert r["self_ms"] > 0 and r["rslv_ms"] > 0
-    assert "evals/root" in out.getvalue()
+        assert r["self_info"] == 0 and r["self_err"] <= TOL and r["self_maxerr"] <= TOL
+        assert r["rslv_err"] <= TOL and r["self_ms"] > 0 and r["rslv_ms"] > 0
+        st = r["merges"]
+        # every root converged within LAPACK's iteration bound (30 steps
+        # after the midpoint and initial evaluations), the deflation counts
+        # add up per merge, and every root of a merge of three or more poles
+        # took at least one evaluation (merges of one or two poles take the
+        # closed form, counted once)
+        assert int(st["failed"].sum()) == 0 and int(st["evals_max"].max()) <= 32
+        assert np.all(st["k"] + st["givens"] + st["tiny"] == st["size"])
+        big = st["k"] > 2
+        assert np.all(st["evals"][big] >= st["k"][big])
+        assert set(r["level_ms"]) == set(range(-1, int(st["level"].max()) + 1))
+    assert "iter_max" in out.getvalue()
