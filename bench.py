@@ -13,11 +13,16 @@ GPU's error there as max |dw| / max(1, max |w|).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 49152]
        [--no-cpu-baseline] [--leaf L] [--eager] [--no-batched] [--no-grad]
-       [--no-vendor] [--no-compare] [--only batched|differentiated|vendor|config3]
+       [--no-vendor] [--no-compare]
+       [--only batched|differentiated|vendor|config3|sizes]
 
-Besides the main line's fields, the JSON line carries the batched workload
-(2048 matrices, solves/s), the differentiated workload (forward + backward
-ms) and rocSOLVER on the same GPU; --only runs one of those sections alone.
+Besides the main line's fields (the value, a roofline record for the
+dominant kernel class and one per class in kernel_rooflines), the JSON line
+carries the same generators at N = 16384 and 32768 and the reference's
+shipped reduced-dense input (ms and error against DSTERF), the batched
+workload (2048 matrices, solves/s, per-class rooflines), the differentiated
+workload (forward + backward ms, backward roofline), rocSOLVER on the same
+GPU, and the N = 65536 solve; --only runs one of those sections alone.
 """
 from __future__ import annotations
 
@@ -35,6 +40,7 @@ sys.path.insert(0, os.path.join(ROOT, "ppopp27-brdc-ae_amd"))
 
 GENS = ("uniform", "normal", "toeplitz", "clustered")
 PEAK_FP64 = 78.6e12  # MI355X FP64 vector (= FP64 matrix) peak, FLOP/s
+PEAK_HBM = 8.0e12    # MI355X HBM3E, bytes/s
 
 
 def progress(msg):
@@ -67,6 +73,35 @@ def profiled_traffic(kernel_class):
                         and parts[1].split("<")[0] == "k_" + kernel_class:
                     return float(parts[2]), os.path.relpath(path, ROOT)
     return None, None
+
+
+FP64_CLASSES = ("secular", "secular_mid", "secular_stream", "wprod", "rows")
+
+
+def class_rooflines(prof):
+    """One roofline record per kernel class of a profile_kernels result: FP64
+    VALU classes against the FP64 peak (flops from the solve's own per-merge
+    K and evaluation counts), memory-bound classes against HBM (algorithmic
+    bytes per row per merge); each with its summed time, launches and
+    average launch time."""
+    out = {}
+    for name, rec in prof["kernels"].items():
+        base = {"ms": round(rec["ms"], 4), "launches": rec["launches"],
+                "avg_launch_us": round(rec["avg_us"], 2)}
+        if name in FP64_CLASSES and prof["flops"].get(name):
+            a = prof["flops"][name] / (rec["ms"] * 1e-3)
+            out[name] = {"bound": "fp64_valu", "achieved": round(a / 1e12, 3),
+                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                         "frac": round(a / PEAK_FP64, 4),
+                         "algorithmic_flops": prof["flops"][name], **base}
+        elif prof["bytes"].get(name):
+            a = prof["bytes"][name] / (rec["ms"] * 1e-3)
+            out[name] = {"bound": "hbm", "achieved": round(a / 1e9, 2), "peak": PEAK_HBM / 1e9,
+                         "unit": "GB/s", "frac": round(a / PEAK_HBM, 5),
+                         "algorithmic_bytes": prof["bytes"][name], **base}
+        else:
+            out[name] = base
+    return out
 
 
 def cpu_baseline(mats, budget_s=25.0):
@@ -105,19 +140,18 @@ def cpu_baseline(mats, budget_s=25.0):
         eigs["toeplitz"] = w
     timed = [per[g] for g in GENS if g in per]
     # LAPACK DSTERF, the reference's acceptance oracle (smoke_br.f90:30), on
-    # one core: O(N^2), so timed on a bounded sample, the N = 16384 normal
-    # matrix (scipy's OpenBLAS, whose DSTERF is sequential)
+    # one core: O(N^2) and sequential (scipy's OpenBLAS), one solve of the
+    # normal matrix of the workload (about 20 s at N = 49152)
     sterf = None
     try:
         from scipy.linalg import lapack
-        from brdc.generators import generate, seed_for
-        ds, es = generate("normal", 16384, seed_for(2, 16384))
+        ds, es = mats["normal"]
         t0 = time.perf_counter()
         _, info_s = lapack.dsterf(ds.copy(), es.copy())
         sterf = {"value": round(1e3 * (time.perf_counter() - t0), 1), "unit": "ms/solve",
                  "cores": 1, "kind": "lapack", "info": int(info_s),
-                 "sample": "LAPACK DSTERF (scipy's OpenBLAS) on the N=16384 normal matrix, "
-                           "one solve"}
+                 "sample": "LAPACK DSTERF (scipy's OpenBLAS) on the N=%d normal matrix, one "
+                           "solve" % len(ds)}
     except ImportError:
         pass
     rec = {"value": float(np.mean(timed)), "unit": "ms/solve", "cores": cores, "kind": "port",
@@ -235,8 +269,20 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True, leaf=0, compar
     infos = bs.info.cpu().numpy()[:bs.nmat]
     ws = bs.split(w)
     inv = max(invariants_err(ws[i], *mats[i]) for i in range(len(mats)))
+    # kernel classes of one batched solve (events around each launch) and the
+    # batch's FP64 work from its own per-merge counts
+    wprof = torch.empty_like(d_dev)
+    bprof = brdc.profile_kernels(bs, [(wprof, d_dev, e_dev)])
+    broof = class_rooflines(bprof)
+    flops = sum(bprof["flops"].values())
+    fp64_ms = sum(bprof["kernels"][k]["ms"] for k in FP64_CLASSES if k in bprof["kernels"])
     rec = {"workload": "2048 tridiagonals, n = round(exp(U(ln 32, ln 4096))), D,E ~ N(0,1), "
                        "one flattened merge forest per launch sequence (hipGraph replay)",
+           "kernel_rooflines": broof,
+           "fp64_flops_per_batch": flops,
+           "fp64_achieved_tflops_whole_batch": round(flops / (ms * 1e-3) / 1e12, 3),
+           "fp64_frac_in_fp64_kernels": round(flops / (fp64_ms * 1e-3) / PEAK_FP64, 4)
+           if fp64_ms > 0 else None,
            "matrices": len(mats), "rows": int(off[-1]), "leaf_max": tune["leaf_max"],
            "autotune_ms_per_batch": {str(k): round(v, 4) for k, v in tune["ms"].items()},
            "autotune_s": round(tune_s, 2),
@@ -356,9 +402,23 @@ def bench_differentiated(steps, warmup, compare=True):
         re = 2.0 * (V[:-1] * V[1:]) @ g_np[o:o + n]
         worst_dense = max(worst_dense, np.max(np.abs(gdv[o:o + n] - rd)),
                           np.max(np.abs(gev[o:o + n - 1] - re)))
+    # the backward's algorithmic FP64 work: per (eigenvalue, row) 18 flops --
+    # the two pivot chains (3 each: subtract, subtract, divide), the twist
+    # element (3), the vector entry (2), its norm (2) and the two
+    # contributions g v^2 and 2 g v v' (5) -- over n^2 pairs per matrix
+    bwd_flops = 18.0 * float(sum(int(n) ** 2 for n in sizes))
+    traffic, traffic_src = profiled_traffic("grad_vectors")
+    bwd_roof = {"bound": "fp64_valu", "kernels": "k_grad_vectors (four launches) + k_grad_reduce",
+                "achieved": round(bwd_flops / (ms_b * 1e-3) / 1e12, 3),
+                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": round(bwd_flops / (ms_b * 1e-3) / PEAK_FP64, 4),
+                "algorithmic_flops": bwd_flops,
+                "algorithmic_io_bytes": 8 * 6 * int(sum(sizes)),
+                "traffic_bytes_per_launch": traffic, "traffic_source": traffic_src}
     return {"workload": "256 tridiagonals at n=1024 + one at n=16384, D,E ~ N(0,1), upstream "
                         "g ~ N(0,1); forward = one batched BR solve, backward = twisted-"
                         "factorization eigenvector kernel (torch.autograd)",
+            "backward_roofline": bwd_roof,
             "fwd_bwd_ms": round(ms_fb, 3), "fwd_ms": round(ms_f, 3), "bwd_ms": round(ms_b, 3),
             "plain_per_matrix_fwd_ms": ms_pm,
             "worst_identity_rel_err": worst_id, "max_abs_err_vs_dense_eigh_grad": worst_dense,
@@ -412,6 +472,59 @@ def bench_config3_and_host_entry(reps=5):
                                      "the eigenvalues", "ms_per_solve_median": round(
                                          float(np.median(times)), 3), "info": int(info2)}
     return rec
+
+
+def bench_other_sizes_and_shipped_input(reps=5):
+    """The plain solves at the other two sizes of the workload family
+    (N = 16384 and 32768, the four generators, seed_for(2, N)) and the
+    reference's shipped input (data/inputs/reduced_dense_49152.de.bin, kept in
+    tests/golden/), each as a graph replay with its inputs in HBM: ms per
+    solve and the normwise error against LAPACK DSTERF (tests/golden/
+    dsterf_config2.npz, reduced_dense_49152_dsterf.npz)."""
+    import torch
+    import brdc
+    from brdc.generators import generate, read_de_bin, seed_for
+    dev = torch.device("cuda:0")
+    gold = os.path.join(ROOT, "tests", "golden")
+    fix = np.load(os.path.join(gold, "dsterf_config2.npz"))
+
+    def run(d, e):
+        dt, et = torch.from_numpy(d).to(dev), torch.from_numpy(e).to(dev)
+        gs = brdc.GraphSolve(brdc.Solver(d.size, device=dev), dt, et)
+        for _ in range(2):
+            gs.run()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            gs.run()
+        b.record()
+        torch.cuda.synchronize()
+        w = gs.w.cpu().numpy()
+        stats = gs.solver.merge_stats()
+        return a.elapsed_time(b) / reps, w, int(gs.solver.info.item()), stats
+
+    out = {}
+    for n in (16384, 32768):
+        rec = {}
+        for g in GENS:
+            d, e = generate(g, n, seed_for(2, n))
+            ms, w, info, _ = run(d, e)
+            rec[g] = {"ms": round(ms, 4), "info": info,
+                      "vs_dsterf": rel_err(w, fix[f"{g}_{n}"])}
+        rec["mean_ms"] = round(float(np.mean([rec[g]["ms"] for g in GENS])), 4)
+        out[str(n)] = rec
+    d, e = read_de_bin(os.path.join(gold, "reduced_dense_49152.de.bin"))
+    ref = np.load(os.path.join(gold, "reduced_dense_49152_dsterf.npz"))["w"]
+    ms, w, info, stats = run(d, e)
+    root = stats[stats["level"] == 0]
+    out["reduced_dense_49152"] = {
+        "workload": "the reference's shipped input data/inputs/reduced_dense_49152.de.bin "
+                    "(tridiagonal of a reduced dense matrix), one solve",
+        "ms": round(ms, 4), "info": info, "vs_dsterf": rel_err(w, ref),
+        "vs_dsterf_2norm": float(np.linalg.norm(w - ref) / np.linalg.norm(ref)),
+        "root_merge_K": int(root["k"][0]) if len(root) else None}
+    return out
 
 
 def bench_vendor(n=4096):
@@ -470,7 +583,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the per-matrix GPU comparisons of the batched and "
                          "differentiated sections")
-    ap.add_argument("--only", choices=("batched", "differentiated", "vendor", "config3"),
+    ap.add_argument("--only", choices=("batched", "differentiated", "vendor", "config3",
+                                       "sizes"),
                     default=None,
                     help="run only this section and print its record as the JSON line "
                          "(for profiling one workload at a time)")
@@ -492,7 +606,8 @@ def main():
                                                 leaf=args.leaf, compare=not args.no_compare),
                "differentiated": lambda: bench_differentiated(args.steps, args.warmup,
                                                               compare=not args.no_compare),
-               "vendor": bench_vendor, "config3": bench_config3_and_host_entry}[args.only]()
+               "vendor": bench_vendor, "config3": bench_config3_and_host_entry,
+               "sizes": bench_other_sizes_and_shipped_input}[args.only]()
         print(json.dumps({"section": args.only, **rec}), flush=True)
         return
 
@@ -628,8 +743,8 @@ def main():
     if dom_name in prof["flops"]:
         flops = prof["flops"][dom_name]
         achieved = flops / (dom_rec["ms"] * 1e-3)
-        roof = {"bound": "mfma", "pipe": "FP64 VALU (no matrix-core op on this path; the FP64 "
-                "vector and FP64 matrix peaks of MI355X are both 78.6 TFLOP/s)",
+        roof = {"bound": "fp64_valu", "pipe": "FP64 vector ALU (no matrix-core op on this path: "
+                "no dense contraction; MI355X's FP64 vector peak is 78.6 TFLOP/s)",
                 "kernel": dom_name, "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": traffic,
                 "traffic_unit": "bytes/launch", "traffic_source": traffic_src,
@@ -669,11 +784,14 @@ def main():
                      "binary128_probes": qp},
         "roofline": roof,
         "kernel_breakdown_ms_per_step": {k: round(v["ms"], 4) for k, v in prof["kernels"].items()},
+        "kernel_rooflines": class_rooflines(prof),
         "cpu_baseline": cpu_rec,
         "wall_ms_timed_region": round(wall_ms, 3),
     }
     if cpu_rec:
         line["speedup_vs_cpu_baseline"] = round(cpu_rec["value"] / value, 2) if value > 0 else None
+    progress("other sizes and the shipped input start")
+    line["other_sizes_and_shipped_input"] = bench_other_sizes_and_shipped_input()
     if not args.no_batched:
         progress("batched section starts")
         line["batched"] = bench_batched(max(3, args.steps // 2), max(1, args.warmup),

@@ -226,15 +226,19 @@ def merge_stats(solver: Solver):
     return solver.merge_stats()
 
 
-#: algorithmic HBM bytes per matrix row per level for the memory-bound kernels
-#: (prepare: read d, blo, bhi + write d, z, q1, q2; deflate: read 4 + write 4
-#: doubles plus flags and indices; finalize: read 3 + write 3 doubles; leaf:
-#: read d, e + write d, blo, bhi)
-_BYTES_PER_ROW = {"prepare": 56, "deflate": 112, "finalize": 48, "leaf": 40}
+#: algorithmic HBM bytes per matrix row for the memory-bound kernels, per
+#: merge the row takes part in (prepare: read d, blo, bhi + write d, z, q1,
+#: q2; deflate: read 4 + write 4 doubles plus flags and indices; finalize:
+#: read 3 + write 3 doubles) or once per solve (leaf: read d, e + write d,
+#: blo, bhi; norm: read d, e; prep_e: read e + write the scaled e)
+_BYTES_PER_ROW = {"prepare": 56, "deflate": 112, "finalize": 48, "leaf": 40, "norm": 16,
+                  "prep_e": 16}
 
 
 def profile_kernels(solver: Solver, problems, stream=None):
-    """Run one solve per (w, d, e) in problems with kernel-class timing on.
+    """Run one solve per (w, d, e) in problems with kernel-class timing on
+    (solver: a Solver, or a BatchSolver whose problems hold the concatenated
+    rows).
 
     Returns {"kernels": {class: {"ms", "launches", "avg_us"}}, "flops": {class:
     FP64 flops}, "bytes": {class: bytes}} summed over the problems, where the
@@ -277,9 +281,11 @@ def profile_kernels(solver: Solver, problems, stream=None):
         nonroot = (stats["level"] > 0) & (stats["k"] > 2)
         flops["wprod"] += float(np.sum(3.0 * k[nonroot] ** 2))
         flops["rows"] += float(np.sum(9.0 * k[nonroot] ** 2))
-        levels = int(stats["level"].max()) + 1 if len(stats) else 0
+        # every merge moves its rows once: the rows of all merges together
+        merged_rows = float(stats["size"].sum())
+        rows = float(getattr(solver, "rows", getattr(solver, "n", 0)))
         for kname, b in _BYTES_PER_ROW.items():
-            nbytes[kname] += b * solver.n * (1 if kname == "leaf" else levels)
+            nbytes[kname] += b * (rows if kname in ("leaf", "norm", "prep_e") else merged_rows)
     kern = {}
     for i, name in enumerate(_lib.KERNEL_CLASSES):
         if tot_n[i]:

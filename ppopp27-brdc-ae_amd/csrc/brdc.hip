@@ -70,7 +70,7 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 // cover `rows` rows (one matrix, or a batch's concatenated rows); the
 // per-matrix, plan and count records follow.
 struct Layout {
-    size_t scal, ew;
+    size_t scal, npart, ew;
     size_t sd[2], slo[2], shi[2];
     size_t md, mz, mq1, mq2;
     size_t ad, az, adz, aq;
@@ -97,6 +97,7 @@ Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items,
         return at;
     };
     L.scal = take(2 * (size_t)std::max(nmat, 1) * sizeof(double));
+    L.npart = take(2 * (size_t)kNormMaxParts * sizeof(double));
     L.ew = take(nd);
     for (int b = 0; b < 2; b++) {
         L.sd[b] = take(nd);
@@ -221,9 +222,14 @@ void launch_wide(K1 k1, K2 k2, K4 k4, K8 k8, int WL, int blocks, hipStream_t st,
 // deflation block size follows the level's node size
 template <class... Args>
 void launch_deflate(int maxsz, int merges, hipStream_t st, Args... args) {
-    if (maxsz <= 512)
+    // each wave walks its share of a merge's positions in 64-wide tiles, one
+    // after another: blocks wide enough for about two tiles a wave, as far
+    // as the level's merges leave room on the device (2^19 threads)
+    int T = (maxsz <= 128) ? 64 : (maxsz <= 512) ? 256 : 1024;
+    while (T > 64 && (long long)merges * T > (1LL << 19)) T /= 4;
+    if (T == 64)
         hipLaunchKernelGGL(k_deflate<64>, dim3(merges), dim3(64), 0, st, args...);
-    else if (maxsz <= 8192)
+    else if (T == 256)
         hipLaunchKernelGGL(k_deflate<256>, dim3(merges), dim3(256), 0, st, args...);
     else
         hipLaunchKernelGGL(k_deflate<1024>, dim3(merges), dim3(1024), 0, st, args...);
@@ -279,17 +285,20 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
     double* ew = at<double>(ws, L.ew);
     int* counts_all = at<int>(ws, L.counts);
 
+    // one large matrix: its norm over several blocks, finished in k_prep_e
+    const int nparts = S.moff ? 0 : std::min(kNormMaxParts, n / kNormRowsPerPart);
+    double* part = at<double>(ws, L.npart);
     {
         ProfScope ps(kNorm, st);
-        hipLaunchKernelGGL(k_norm, dim3(S.nmat), dim3(1024), 0, st, n, S.moff, d, e, scal, info,
-                           info_user);
+        hipLaunchKernelGGL(k_norm, dim3(nparts > 1 ? nparts : S.nmat), dim3(1024), 0, st, n,
+                           S.moff, d, e, scal, info, info_user, part);
     }
     BRDC_LAUNCH_CHECK();
     if (n <= 1) return BRDC_STATUS_SUCCESS;
     {
         ProfScope ps(kPrepE, st);
         hipLaunchKernelGGL(k_prep_e, dim3((n + 255) / 256), dim3(256), 0, st, n, S.moff, S.nmat,
-                           d, e, scal, ew);
+                           d, e, scal, ew, nparts > 1 ? nparts : 0, (const double*)part, info);
     }
     BRDC_LAUNCH_CHECK();
 

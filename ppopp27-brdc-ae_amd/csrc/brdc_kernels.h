@@ -152,28 +152,54 @@ __device__ __forceinline__ void report_item_failure(const FailSite& f, int size)
 // ---------------------------------------------------------------------------
 // max-norm, finiteness and scale (DLANST('M') + DLASCL, dstedc_dc_br.f:86-118)
 // One block per matrix (moff: first row of each matrix; nullptr: one matrix
-// of n rows).
+// of n rows), or, for one large matrix, nparts blocks over row slices that
+// leave (max, non-finite flag) partials in part[2b], part[2b+1]; k_prep_e
+// then forms the scale from them.
+constexpr int kNormRowsPerPart = 4096;
+constexpr int kNormMaxParts = 32;
+
+__device__ __forceinline__ void norm_finish(double mm, bool bad, int nb, double* scal, int* info,
+                                            int b, const int* moff, int* info_user) {
+    const bool ok = mm > 0.0;
+    scal[2 * b] = ok ? 1.0 / mm : 1.0;  // scale into [-1, 1]
+    scal[2 * b + 1] = ok ? mm : 1.0;    // undo
+    info[moff ? b : 0] = bad ? nb + 1 : 0;
+    if (info_user && nb <= 1) *info_user = info[0];
+}
+
 __global__ __launch_bounds__(1024) void k_norm(int n, const int* __restrict__ moff,
                                                const double* __restrict__ d,
                                                const double* __restrict__ e, double* scal,
-                                               int* info, int* info_user) {
+                                               int* info, int* info_user, double* part) {
     __shared__ double red[16];
     __shared__ int bad[16];
     const int b = blockIdx.x;
+    const int nparts = gridDim.x;
+    const bool split = (moff == nullptr && nparts > 1);
     const int off = moff ? moff[b] : 0;
     const int nb = moff ? moff[b + 1] - off : n;
+    // rows [lo, hi) of the matrix (all of it, or this block's slice)
+    const int lo = split ? (int)((long long)nb * b / nparts) : 0;
+    const int hi = split ? (int)((long long)nb * (b + 1) / nparts) : nb;
     d += off;
     e += off;
     double m = 0.0;
     int nonfinite = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-        const double x = d[i];
-        nonfinite |= !isfinite(x);
-        m = fmax(m, fabs(x));
-        if (i < nb - 1) {
-            const double y = e[i];
-            nonfinite |= !isfinite(y);
-            m = fmax(m, fabs(y));
+    // eight rows per thread and step, all loads issued before the maxima, so
+    // one block keeps many loads in flight
+    constexpr int kU = 8;
+    for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += kU * blockDim.x) {
+        double x[kU], y[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            x[u] = (i < hi) ? d[i] : 0.0;
+            y[u] = (i < hi && i < nb - 1) ? e[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            nonfinite |= !isfinite(x[u]) | !isfinite(y[u]);
+            m = fmax(m, fmax(fabs(x[u]), fabs(y[u])));
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -193,11 +219,12 @@ __global__ __launch_bounds__(1024) void k_norm(int n, const int* __restrict__ mo
             mm = fmax(mm, red[w]);
             bb |= bad[w];
         }
-        const bool ok = mm > 0.0;
-        scal[2 * b] = ok ? 1.0 / mm : 1.0;  // scale into [-1, 1]
-        scal[2 * b + 1] = ok ? mm : 1.0;    // undo
-        info[moff ? b : 0] = bb ? nb + 1 : 0;
-        if (info_user && nb <= 1) *info_user = info[0];
+        if (split) {
+            part[2 * b] = mm;
+            part[2 * b + 1] = bb ? 1.0 : 0.0;
+        } else {
+            norm_finish(mm, bb != 0, nb, scal, info, b, moff, info_user);
+        }
     }
 }
 
@@ -206,10 +233,25 @@ __global__ __launch_bounds__(1024) void k_norm(int n, const int* __restrict__ mo
 // merge (everything deflates) or inside a leaf
 // Every row gets an entry; the last row of each matrix gets 0, so a batch's
 // matrices are decoupled wherever a leaf or merge reads across their border.
+// With nparts > 0 (one matrix whose norm k_norm left in nparts partials),
+// every block forms the scale from the partials itself and block 0 stores it
+// with INFO for the kernels that follow.
 __global__ void k_prep_e(int n, const int* __restrict__ moff, int nmat,
                          const double* __restrict__ d, const double* __restrict__ e,
-                         const double* __restrict__ scal, double* ew) {
+                         double* scal, double* ew, int nparts, const double* __restrict__ part,
+                         int* info) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sc;
+    if (nparts > 0) {
+        double mm = 0.0;
+        bool bad = false;
+        for (int q = 0; q < nparts; q++) {
+            mm = fmax(mm, part[2 * q]);
+            bad = bad || part[2 * q + 1] != 0.0;
+        }
+        sc = (mm > 0.0) ? 1.0 / mm : 1.0;
+        if (i == 0) norm_finish(mm, bad, n, scal, info, 0, nullptr, nullptr);
+    }
     if (i >= n) return;
     int b = 0, last = n - 1;
     if (moff) {  // the matrix holding row i: moff[b] <= i < moff[b + 1]
@@ -228,7 +270,8 @@ __global__ void k_prep_e(int n, const int* __restrict__ moff, int nmat,
     }
     const double tiny = kEps * sqrt(fabs(d[i])) * sqrt(fabs(d[i + 1]));
     const double x = e[i];
-    ew[i] = (fabs(x) > tiny) ? x * scal[2 * b] : 0.0;
+    if (nparts <= 0) sc = scal[2 * b];
+    ew[i] = (fabs(x) > tiny) ? x * sc : 0.0;
 }
 
 // ---------------------------------------------------------------------------
