@@ -8,6 +8,15 @@ by twisted factorization at its eigenvalue; nothing is stored by the forward
 pass beyond D, E and the eigenvalues).  ``eigvalsh_tridiagonal_batched``
 does the same for a batch's concatenated rows through one merge forest.
 The reference solver is values-only and has no backward; this layer is new.
+
+Repeated eigenvalues: eigenvalues within 2^-44 ||T|| of a neighbour (a
+matrix split into identical blocks, exact multiplicities) get their vectors
+from inverse iteration with Gram-Schmidt inside the cluster, so the cluster's
+vectors are orthonormal; only losses that give a cluster's members equal
+weight have a gradient there, and for them the result is exact to rounding.
+Pairs farther apart than that but closer than about 1e-8 ||T|| whose vectors
+overlap (Wilkinson's W21+ below its top pair) keep twisted-factorization
+vectors, accurate to about eps ||T|| / gap.
 """
 from __future__ import annotations
 

@@ -514,13 +514,16 @@ namespace {
 // doubles (grad_cp_slot); per task 64 twist indices and two |z|^2 halves; per gamma item 64
 // best |gamma| values and rows; then the per-task partial sums.
 struct GradShape {
-    size_t cp, rarr, ndarr, nuarr, garr, rgarr, partial, total;
+    size_t cp, rarr, ndarr, nuarr, garr, rgarr, partial, gm, tflag, clus, total;
 };
 
 // checkpoint doubles of one task of an n-row matrix
 inline long long grad_slot(int n) { return grad_cp_slot(n); }
 
-GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles, int ngamma) {
+// rows: all rows of the call; nmax: its largest matrix (the cluster kernel's
+// per-wave arrays)
+GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles, int ngamma,
+                     int rows, int nmax) {
     GradShape G{};
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -537,6 +540,10 @@ GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles,
     G.garr = take(glanes * sizeof(double));
     G.rgarr = take(glanes * sizeof(int));
     G.partial = take((size_t)std::max<long long>(partial_doubles, 1) * sizeof(double));
+    G.gm = take((size_t)std::max(rows, 1) * sizeof(double));
+    G.tflag = take((size_t)std::max(ntask, 1) * sizeof(int));
+    G.clus = take((size_t)kClusterWaves * kClusterArrays * (size_t)std::max(nmax, 1) *
+                  sizeof(double));
     G.total = off;
     return G;
 }
@@ -556,12 +563,12 @@ double grad_scan_limit() {
     return (end != s && v >= 0.0 && v <= kScanBig) ? v : kScanBig;
 }
 
-int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const long long* poff,
-                 const long long* cpoff, const int4* items, const int count[4], int parts,
-                 int ntask, long long cp_doubles, long long partial_doubles, const double* d,
-                 const double* e, const double* w, const double* g, double* dd, double* de,
-                 void* scratch, hipStream_t st) {
-    const GradShape G = grad_shape(ntask, cp_doubles, partial_doubles, count[1]);
+int enqueue_grad(int rows, int nmat, int nmax, const int* moff, const int* tprefix,
+                 const long long* poff, const long long* cpoff, const int4* items,
+                 const int count[4], int parts, int ntask, long long cp_doubles,
+                 long long partial_doubles, const double* d, const double* e, const double* w,
+                 const double* g, double* dd, double* de, void* scratch, hipStream_t st) {
+    const GradShape G = grad_shape(ntask, cp_doubles, partial_doubles, count[1], rows, nmax);
     double* cp = at<double>(scratch, G.cp);
     int* rarr = at<int>(scratch, G.rarr);
     double* ndarr = at<double>(scratch, G.ndarr);
@@ -569,9 +576,16 @@ int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const 
     double* garr = at<double>(scratch, G.garr);
     int* rgarr = at<int>(scratch, G.rgarr);
     double* part = at<double>(scratch, G.partial);
+    double* gm = at<double>(scratch, G.gm);
     const int n_one = moff ? 0 : rows;
     if (ntask <= 0) return BRDC_STATUS_SUCCESS;
     const double scan_big = grad_scan_limit();
+    // members of numerically degenerate clusters leave the twisted
+    // factorization (weight 0 there) for the cluster kernel
+    int* tflag = at<int>(scratch, G.tflag);
+    hipLaunchKernelGGL(k_grad_mask, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat, moff,
+                       tprefix, w, g, gm, tflag);
+    BRDC_LAUNCH_CHECK();
     size_t base = 0;
     for (int l = 0; l < 4; l++) {
         const int cnt = count[l];
@@ -580,12 +594,17 @@ int enqueue_grad(int rows, int nmat, const int* moff, const int* tprefix, const 
             hipLaunchKernelGGL(k_grad_vectors, dim3((waves + kGradWaves - 1) / kGradWaves),
                                dim3(kWave * kGradWaves), 0, st, l + 1,
                                items ? items + base : (const int4*)nullptr, cnt, parts, nmat, moff,
-                               tprefix, poff, cpoff, n_one, d, e, w, g, cp, rarr, ndarr, nuarr,
-                               garr, rgarr, part, scan_big);
+                               tprefix, poff, cpoff, n_one, d, e, w, (const double*)gm, cp, rarr,
+                               ndarr, nuarr, garr, rgarr, part, scan_big);
             BRDC_LAUNCH_CHECK();
         }
         base += (size_t)std::max(cnt, 0);
     }
+    hipLaunchKernelGGL(k_grad_clusters,
+                       dim3(std::min((ntask + kWave - 1) / kWave, kClusterWaves)), dim3(kWave), 0,
+                       st, ntask, nmat, moff, tprefix, poff, n_one, nmax, d, e, w, g,
+                       (const int*)tflag, at<double>(scratch, G.clus), part);
+    BRDC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
                        moff, tprefix, poff, n_one, (const double*)part, dd, de);
     BRDC_LAUNCH_CHECK();
@@ -1051,7 +1070,8 @@ int brdc_dstebr_grad_workspace(int n, size_t* bytes) {
     const int tasks = (n + kWave - 1) / kWave;
     int count[4];
     grad_counts_one(n, count);
-    *bytes = grad_shape(tasks, tasks * grad_slot(n), (long long)tasks * 2 * n, count[1]).total;
+    *bytes = grad_shape(tasks, tasks * grad_slot(n), (long long)tasks * 2 * n, count[1], n, n)
+                 .total;
     return BRDC_STATUS_SUCCESS;
 }
 
@@ -1066,10 +1086,10 @@ int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
     const long long partial = (long long)tasks * 2 * n, cpd = tasks * grad_slot(n);
     int count[4];
     grad_counts_one(n, count);
-    if (scratch_bytes < grad_shape(tasks, cpd, partial, count[1]).total ||
+    if (scratch_bytes < grad_shape(tasks, cpd, partial, count[1], n, n).total ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
-    return enqueue_grad(n, 1, nullptr, nullptr, nullptr, nullptr, nullptr, count, grad_parts(n),
+    return enqueue_grad(n, 1, n, nullptr, nullptr, nullptr, nullptr, nullptr, count, grad_parts(n),
                         tasks, cpd, partial, d_dev, e_dev, w_dev, g_dev, dd_dev, de_dev,
                         scratch, static_cast<hipStream_t>(stream));
 }
@@ -1077,7 +1097,7 @@ int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
 int brdc_batch_grad_workspace(const brdc_batch_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return BRDC_STATUS_INVALID_POINTER;
     *bytes = grad_shape(plan->tprefix.back(), plan->cpoff.back(), plan->poff.back(),
-                        plan->gcount[1]).total;
+                        plan->gcount[1], plan->rows, plan->rows_max).total;
     return BRDC_STATUS_SUCCESS;
 }
 
@@ -1093,11 +1113,13 @@ int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const doub
     const int ntask = plan->tprefix.back();
     const long long partial = plan->poff.back();
     if (workspace_bytes < L.total ||
-        scratch_bytes < grad_shape(ntask, plan->cpoff.back(), partial, plan->gcount[1]).total ||
+        scratch_bytes < grad_shape(ntask, plan->cpoff.back(), partial, plan->gcount[1],
+                                   plan->rows, plan->rows_max).total ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
     void* ws = const_cast<void*>(workspace);
-    return enqueue_grad(plan->rows, plan->nmat, at<int>(ws, L.moff), at<int>(ws, L.tpre),
+    return enqueue_grad(plan->rows, plan->nmat, plan->rows_max, at<int>(ws, L.moff),
+                        at<int>(ws, L.tpre),
                         at<long long>(ws, L.poff), at<long long>(ws, L.cpof),
                         at<int4>(ws, L.gitm), plan->gcount, 1, ntask, plan->cpoff.back(), partial,
                         d_dev, e_dev,

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         auto at = [&](int s) { return (long long)s * kWave + lane; };
         // D and E reach the segments through a window of kWin rows in this
         // wave's LDS (refilled with coalesced loads when a segment leaves it;
-        // one wave's LDS operations complete in order, so no barrier).  A
+        // wave barriers order the refill's stores against the reads).  A
         // segment reads D of rows lo..lo+16 and E of rows lo-1..lo+15: D
         // clamped to the matrix, E zero outside rows 0..n-2 (rows outside the
         // matrix only feed values that are never used, except that the zero E
@@ -257,11 +257,16 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 const double x = e[off + min(max(row, 0), n - 2)];
                 ve[q] = (row >= 0 && row <= n - 2) ? x : 0.0;
             }
+            // lanes read entries other lanes stored: the wave's earlier reads
+            // of the window finish before these stores, and the stores before
+            // the reads that follow
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
                 sD[lane + q * kWave] = vd[q];
                 sE[lane + q * kWave] = ve[q];
             }
+            __builtin_amdgcn_wave_barrier();
         };
         double dv[kSeg + 1], ev[kSeg], eprev;
         auto load_seg = [&](int lo, bool up) {
@@ -655,6 +660,262 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 if (row < len) part[(idx < kSeg ? 0 : n) + lo + row] = total;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Numerically degenerate eigenvalues.  Twisted factorization at an eigenvalue
+// of a cluster whose members agree to within rounding returns one vector for
+// all of them (a matrix split into identical blocks: both copies of lambda
+// pick the same twist), and those vectors are neither distinct nor
+// orthogonal.  Consecutive eigenvalues closer than kClusterRel * ||T||
+// (||T|| = max |lambda|) form a cluster; its members are taken out of the
+// twisted-factorization pass (their weight g_i is zeroed there) and their
+// vectors are recomputed by inverse iteration with Gram-Schmidt against the
+// cluster's earlier members -- the method of LAPACK DSTEIN: LU with partial
+// pivoting of T - x I (the DLAGTF role), solves with small pivots perturbed
+// (the DLAGTS role), start vectors from a fixed hash, shifts of equal
+// members spread by 10 eps |x|.  Their contributions are added to the partial
+// sums of the task holding the cluster's first member, one wave per task, so
+// the sums stay deterministic.  A cluster of more than kClusterMax members is
+// orthogonalised in groups of kClusterMax.
+constexpr double kClusterRel = 0x1p-44;
+constexpr int kClusterMax = 32;
+constexpr int kClusterWaves = 32;   // waves (one-wave blocks) of the cluster kernel
+constexpr int kClusterIters = 4;    // inverse-iteration solves per member
+constexpr int kClusterArrays = 6 + kClusterMax;  // a, b, c, d, x, pivots, vectors
+
+// this matrix's cluster tolerance from its sorted eigenvalues
+__device__ __forceinline__ double cluster_tol(const double* __restrict__ w, int n) {
+    return kClusterRel * fmax(fabs(w[0]), fabs(w[n - 1]));
+}
+
+// Is eigenvalue k (of n, sorted, w the matrix's own) the first member of a
+// cluster?
+__device__ __forceinline__ bool cluster_starts(const double* __restrict__ w, int n, int k,
+                                               double tol) {
+    return (k == 0 || w[k] - w[k - 1] > tol) && (k + 1 < n && w[k + 1] - w[k] <= tol);
+}
+
+// gm[j] = g[j], or 0 for a member of a cluster (rows numbered as in the
+// batch layout; moff == nullptr: one matrix of rows rows); tflag[t] = 1 when
+// a cluster starts among task t's 64 eigenvalues (written by the thread of
+// the task's first row)
+__global__ void k_grad_mask(int rows, int nmat, const int* __restrict__ moff,
+                            const int* __restrict__ tprefix, const double* __restrict__ w,
+                            const double* __restrict__ g, double* __restrict__ gm,
+                            int* __restrict__ tflag) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= rows) return;
+    int off = 0, n = rows, t0 = 0;
+    if (moff) {
+        int lo = 0, hi = nmat;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (moff[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        off = moff[lo];
+        n = moff[lo + 1] - off;
+        t0 = tprefix[lo];
+    }
+    const int k = j - off;
+    const double* W = w + off;
+    const double tol = cluster_tol(W, n);
+    const bool member = (k > 0 && W[k] - W[k - 1] <= tol) || (k + 1 < n && W[k + 1] - W[k] <= tol);
+    gm[j] = member ? 0.0 : g[j];
+    if ((k & (kWave - 1)) == 0) {
+        bool any = false;
+        for (int q = k; q < min(k + kWave, n); q++) any = any || cluster_starts(W, n, q, tol);
+        tflag[t0 + k / kWave] = any ? 1 : 0;
+    }
+}
+
+// uniform in (-1, 1) from (row, member): the start vector of an iteration
+__device__ __forceinline__ double cluster_start(int k, int m) {
+    unsigned x = (unsigned)k * 0x9E3779B1u ^ ((unsigned)m + 0x7F4A7C15u) * 0x85EBCA77u;
+    x ^= x >> 15;
+    x *= 0x2C1B3C6Du;
+    x ^= x >> 12;
+    x *= 0x297A2D39u;
+    x ^= x >> 15;
+    return ((double)(x >> 8) + 0.5) * 0x1p-23 - 1.0;
+}
+
+// One wave per task with a cluster (tflag), every cluster whose first member
+// lies in the task.  Wave x looks at the tasks 64 at a time, tasks [64 x, 64 x
+// + 64), then 64 (x + waves) on.  Lane 0 runs the sequential factorization
+// and solves; the wave shares the dot products, normalisations and
+// contributions.  scratch: kClusterArrays * nmax doubles per wave.
+__global__ __launch_bounds__(kWave) void k_grad_clusters(
+    int ntask, int nmat, const int* __restrict__ moff, const int* __restrict__ tprefix,
+    const long long* __restrict__ poff, int n_one, int nmax, const double* __restrict__ d,
+    const double* __restrict__ e, const double* __restrict__ w, const double* __restrict__ g,
+    const int* __restrict__ tflag, double* __restrict__ scratch, double* __restrict__ partial) {
+    const int lane = threadIdx.x;
+    double* const la = scratch + (size_t)blockIdx.x * kClusterArrays * nmax;  // U diagonal
+    double* const lb = la + nmax;       // U first superdiagonal
+    double* const lc = lb + nmax;       // multipliers
+    double* const ld = lc + nmax;       // U second superdiagonal
+    double* const lx = ld + nmax;       // the iterate
+    int* const lp = (int*)(lx + nmax);  // row-swap flags (ints in a double-sized slot)
+    double* const vec = lx + 2 * (size_t)nmax;  // the group's finished vectors
+    for (int tb = blockIdx.x * kWave; tb < ntask; tb += gridDim.x * kWave) {
+      unsigned long long todo = __ballot(tb + lane < ntask && tflag[tb + lane] != 0);
+      while (todo) {
+        const int t = tb + __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const int b = uniform(grad_task_matrix(tprefix, nmat, t));
+        const int off = moff ? uniform(moff[b]) : 0;
+        const int n = moff ? uniform(moff[b + 1]) - off : n_one;
+        const int t0 = tprefix ? uniform(tprefix[b]) : 0;
+        if (n < 2) continue;
+        const double* __restrict__ W = w + off;
+        const double* __restrict__ D = d + off;
+        const double* __restrict__ E = e + off;
+        double* __restrict__ part = partial + (poff ? poff[b] : 0) + (long long)(t - t0) * 2 * n;
+        const double tol = cluster_tol(W, n);
+        double tnorm = 0.0;  // max-row-sum norm, for the pivot threshold
+        for (int k = lane; k < n; k += kWave) {
+            const double s = fabs(D[k]) + (k > 0 ? fabs(E[k - 1]) : 0.0) +
+                             (k + 1 < n ? fabs(E[k]) : 0.0);
+            tnorm = fmax(tnorm, s);
+        }
+        for (int o = 32; o > 0; o >>= 1) tnorm = fmax(tnorm, __shfl_xor(tnorm, o, kWave));
+        const double ptol = fmax(kEps * tnorm, 0x1p-1022);
+        const int i0 = (t - t0) * kWave, i1 = min(i0 + kWave, n);
+        for (int cs = i0; cs < i1; cs++) {
+            if (!cluster_starts(W, n, cs, tol)) continue;
+            int ce = cs + 1;
+            while (ce + 1 < n && W[ce + 1] - W[ce] <= tol) ce++;
+            double xprev = 0.0;
+            for (int m = cs; m <= ce; m++) {
+                const int grp = (m - cs) % kClusterMax;  // position in its group
+                if (tnorm == 0.0) {
+                    // T = 0: every vector is an eigenvector; the unit vectors
+                    // e_(m - cs) make the cluster's basis orthonormal
+                    const double gmv = g[off + m];
+                    if (lane == 0) part[(m - cs) % n] += gmv;
+                    continue;
+                }
+                // the shift: equal members spread by 10 eps |x| (DSTEIN)
+                double x = W[m];
+                if (m > cs) x = fmax(x, xprev + 10.0 * kEps * fabs(x));
+                xprev = x;
+                if (lane == 0) {
+                    // LU of T - x I with partial pivoting: entering step k,
+                    // row k holds a0 (column k) and b0 (column k + 1); the
+                    // pivot is the larger of a0 and E_k relative to their
+                    // rows (DLAGTF's rule).  U gets la (diagonal), lb, ld
+                    // (super-diagonals), L the multipliers lc and swaps lp.
+                    double a0 = D[0] - x, b0 = E[0];
+                    double s1 = fabs(a0) + fabs(b0);
+                    for (int k = 0; k < n - 1; k++) {
+                        const double a1 = D[k + 1] - x, c = E[k];
+                        const double bn = (k + 1 < n - 1) ? E[k + 1] : 0.0;
+                        const double s2 = fabs(c) + fabs(a1) + fabs(bn);
+                        const double piv1 = (a0 == 0.0) ? 0.0 : fabs(a0) / s1;
+                        const double piv2 = (c == 0.0) ? 0.0 : fabs(c) / s2;
+                        if (c == 0.0 || piv2 <= piv1) {
+                            const double mult = (c == 0.0) ? 0.0 : c / a0;
+                            la[k] = a0;
+                            lb[k] = b0;
+                            ld[k] = 0.0;
+                            lc[k] = mult;
+                            lp[k] = 0;
+                            a0 = a1 - mult * b0;
+                            b0 = bn;
+                            s1 = s2;
+                        } else {
+                            const double mult = a0 / c;
+                            la[k] = c;
+                            lb[k] = a1;
+                            ld[k] = bn;
+                            lc[k] = mult;
+                            lp[k] = 1;
+                            a0 = b0 - mult * a1;
+                            b0 = -mult * bn;
+                        }
+                    }
+                    la[n - 1] = a0;
+                }
+                // start vector
+                for (int k = lane; k < n; k += kWave) lx[k] = cluster_start(k, m - cs);
+                __builtin_amdgcn_wave_barrier();
+                __threadfence();
+                for (int it = 0; it < kClusterIters; it++) {
+                    // right-hand side of size ptol: the solution's entries stay
+                    // near 1 however small the pivots, so nothing overflows
+                    for (int k = lane; k < n; k += kWave) lx[k] *= ptol;
+                    __builtin_amdgcn_wave_barrier();
+                    __threadfence();
+                    if (lane == 0) {
+                        // forward: apply the row swaps and multipliers
+                        for (int k = 0; k < n - 1; k++) {
+                            if (lp[k]) {
+                                const double tk = lx[k];
+                                lx[k] = lx[k + 1];
+                                lx[k + 1] = tk - lc[k] * lx[k + 1];
+                            } else {
+                                lx[k + 1] -= lc[k] * lx[k];
+                            }
+                        }
+                        // back substitution, pivots below ptol perturbed
+                        for (int k = n - 1; k >= 0; k--) {
+                            double tmp = lx[k];
+                            if (k + 1 < n) tmp -= lb[k] * lx[k + 1];
+                            if (k + 2 < n) tmp -= ld[k] * lx[k + 2];
+                            double ak = la[k];
+                            if (fabs(ak) < ptol) ak = copysign(ptol, ak);
+                            lx[k] = tmp / ak;
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    __threadfence();
+                    // Gram-Schmidt against the group's earlier vectors, twice
+                    for (int pass = 0; pass < 2; pass++) {
+                        for (int q = 0; q < grp; q++) {
+                            const double* v = vec + (size_t)q * nmax;
+                            double dot = 0.0;
+                            for (int k = lane; k < n; k += kWave) dot = fma(v[k], lx[k], dot);
+                            for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, kWave);
+                            for (int k = lane; k < n; k += kWave) lx[k] = fma(-dot, v[k], lx[k]);
+                        }
+                    }
+                    // normalise: scale by the largest entry first, then the 2-norm
+                    double amax = 0.0;
+                    for (int k = lane; k < n; k += kWave) amax = fmax(amax, fabs(lx[k]));
+                    for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, kWave));
+                    const double inv = (amax > 0.0) ? 1.0 / amax : 1.0;
+                    double ss = 0.0;
+                    for (int k = lane; k < n; k += kWave) {
+                        const double y = lx[k] * inv;
+                        lx[k] = y;
+                        ss = fma(y, y, ss);
+                    }
+                    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, kWave);
+                    const double rn = 1.0 / sqrt(ss);
+                    for (int k = lane; k < n; k += kWave) lx[k] *= rn;
+                    __builtin_amdgcn_wave_barrier();
+                    __threadfence();
+                }
+                // keep the vector for the group's later members, and add
+                // g_m v_k^2 and 2 g_m v_k v_{k+1} to the task's partial sums
+                const double gmv = g[off + m];
+                double* v = vec + (size_t)grp * nmax;
+                for (int k = lane; k < n; k += kWave) v[k] = lx[k];
+                __builtin_amdgcn_wave_barrier();
+                __threadfence();
+                for (int k = lane; k < n; k += kWave) {
+                    const double vk = v[k];
+                    part[k] = fma(gmv * vk, vk, part[k]);
+                    if (k + 1 < n) part[n + k] = fma(2.0 * gmv * vk, v[k + 1], part[n + k]);
+                }
+                __builtin_amdgcn_wave_barrier();
+                __threadfence();
+            }
+        }
+      }
     }
 }
 

@@ -310,6 +310,30 @@ def test_splits_zero_matrix_and_small(orc):
         assert info == 0 and rel_err(w, np.linalg.eigvalsh(T)) < 1e-14
 
 
+def test_split_blocks_keep_their_own_scale(orc):
+    """dstedc_dc_br.f:86-140 scales each split block by its own norm (and
+    sends blocks of 25 rows or fewer to DSTERF): a block of size 1e-200 next
+    to one of size 1e150 keeps full relative accuracy in its eigenvalues,
+    checked componentwise against the block alone (scaled to 1 and back, which
+    is exact in binary floating point for these powers of ten up to rounding
+    of one multiply)."""
+    from scipy.linalg import eigh_tridiagonal
+    big_d, big_e = generate("normal", 300, 21)
+    for m in (20, 200):  # a DSTERF-sized block and a D&C-sized one
+        sm_d, sm_e = generate("uniform", m, 22)
+        d = np.concatenate([big_d * 1e150, sm_d * 1e-200])
+        e = np.concatenate([big_e * 1e150, [0.0], sm_e * 1e-200])
+        w, info = orc.dstebr(d, e)
+        assert info == 0 and np.all(np.diff(w) >= 0)
+        small = np.sort(w[np.abs(w) < 1e-100])
+        ref = np.sort(eigh_tridiagonal(sm_d, sm_e, eigvals_only=True)) * 1e-200
+        assert small.shape == (m,)
+        assert np.max(np.abs(small - ref) / np.max(np.abs(ref))) < 1e-13
+        large = np.sort(w[np.abs(w) >= 1e-100])
+        refl = np.sort(eigh_tridiagonal(big_d, big_e, eigvals_only=True)) * 1e150
+        assert rel_err(large, refl) < 1e-13
+
+
 def test_sterf_matches_dsterf_fixtures(orc):
     """The oracle's values-only QL (the DSTERF role) against LAPACK DSTERF."""
     g = np.load(os.path.join(GOLD, "dsterf_eigs.npz"))
