@@ -402,9 +402,18 @@ __global__ __launch_bounds__(LP) void k_leaf(
             // stored by the step before).
             int nxt = mm;
             double dnx = 0.0;
+            // entry k's loads are issued one step ahead: the step stores
+            // entries k + 1 only, so the prefetched entries are current
+            double ek_n = se_(mm - 1), dk_n = sd_(mm - 1), x0_n = s0_(mm - 1), y0_n = s1_(mm - 1);
             for (int k = mm - 1; k >= l; k--) {
-                const double ek = se_(k), dk = sd_(k);
-                const double x0 = s0_(k), y0 = s1_(k);
+                const double ek = ek_n, dk = dk_n;
+                const double x0 = x0_n, y0 = y0_n;
+                if (k > l) {
+                    ek_n = se_(k - 1);
+                    dk_n = sd_(k - 1);
+                    x0_n = s0_(k - 1);
+                    y0_n = s1_(k - 1);
+                }
                 const double f = sn * ek, b = c * ek;
                 double cn, sv;
                 ql_rotation(f, gg, r, cn, sv);
