@@ -12,7 +12,7 @@ leg also bisects 8 eigenvalues per generator in binary128 and reports the
 GPU's error there as max |dw| / max(1, max |w|).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 49152]
-       [--no-cpu-baseline] [--leaf L] [--eager] [--no-batched] [--no-grad]
+       [--no-cpu-baseline] [--leaf L] [--no-tune] [--eager] [--no-batched] [--no-grad]
        [--no-vendor] [--no-compare]
        [--only batched|differentiated|vendor|config3|sizes]
 
@@ -203,14 +203,14 @@ def copy_probe(mib):
                       "launches": 3}), flush=True)
 
 
-def per_matrix_comparison(mats, sizes, ws, ms, leaf):
+def per_matrix_comparison(mats, sizes, ws, ms, opts):
     """The plain per-matrix GPU path on the batch's matrices (one solve per
     matrix, eager launches): time for the whole batch and the largest
     difference from the batched results ws (every 64th matrix)."""
     import torch
     import brdc
     dev = torch.device("cuda:0")
-    solvers = {n: brdc.Solver(n, device=dev, leaf_max=leaf) for n in set(int(x) for x in sizes)}
+    solvers = {n: brdc.Solver(n, device=dev, **opts) for n in set(int(x) for x in sizes)}
     dd = [torch.from_numpy(d).to(dev) for d, _ in mats]
     ee = [torch.from_numpy(e).to(dev) for _, e in mats]
     outs = [torch.empty_like(x) for x in dd]
@@ -250,9 +250,9 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True, leaf=0, compar
     e_dev = torch.from_numpy(e_cat).to(dev)
     t0 = time.perf_counter()
     tune = autotune.tune_batched(d_dev, e_dev, sizes, reps=3) if leaf <= 0 else \
-        {"leaf_max": leaf, "ms": {}}
+        {"leaf_max": leaf, "options": {"leaf_max": leaf}, "ms": {}}
     tune_s = time.perf_counter() - t0
-    bs = brdc.BatchSolver(sizes, device=dev, leaf_max=tune["leaf_max"])
+    bs = brdc.BatchSolver(sizes, device=dev, **tune["options"])
     gs = brdc.GraphSolve(bs, d_dev, e_dev)
     for _ in range(warmup):
         gs.run()
@@ -284,12 +284,13 @@ def bench_batched(steps, warmup, cpu_budget_s=6.0, want_cpu=True, leaf=0, compar
            "fp64_frac_in_fp64_kernels": round(flops / (fp64_ms * 1e-3) / PEAK_FP64, 4)
            if fp64_ms > 0 else None,
            "matrices": len(mats), "rows": int(off[-1]), "leaf_max": tune["leaf_max"],
+           "tuned_options": tune["options"],
            "autotune_ms_per_batch": {str(k): round(v, 4) for k, v in tune["ms"].items()},
            "autotune_s": round(tune_s, 2),
            "ms_per_batch": round(ms, 4), "solves_per_s": round(len(mats) / (ms * 1e-3), 1),
            "info_nonzero": int(np.count_nonzero(infos)), "worst_invariants": inv}
     if compare:
-        rec.update(per_matrix_comparison(mats, sizes, ws, ms, tune["leaf_max"]))
+        rec.update(per_matrix_comparison(mats, sizes, ws, ms, tune["options"]))
     if want_cpu:
         import oracle
         if not os.path.exists(oracle.LIB_PATH):
@@ -572,6 +573,8 @@ def main():
     ap.add_argument("--n", type=int, default=49152)
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch shape instead of autotuning it on the workload")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying a captured graph")
     ap.add_argument("--no-batched", action="store_true",
@@ -621,14 +624,26 @@ def main():
     d_dev = {g: torch.from_numpy(mats[g][0]).to(dev) for g in GENS}
     e_dev = {g: torch.from_numpy(mats[g][1]).to(dev) for g in GENS}
     w_dev = {g: torch.empty_like(d_dev[g]) for g in GENS}
-    solver = brdc.Solver(n, device=dev, leaf_max=args.leaf)
+    # launch shape: autotuned on the step's four matrices (setup, not timed),
+    # or the defaults with a fixed leaf bound when --leaf is given
+    t_tune = time.perf_counter()
+    if args.leaf > 0 or args.no_tune:
+        topts = {"leaf_max": args.leaf}
+        tune_ms = {}
+    else:
+        from brdc import autotune
+        tres = autotune.tune_single([(d_dev[g], e_dev[g]) for g in GENS], reps=3)
+        topts, tune_ms = tres["options"], tres["ms"]
+    tune_s = time.perf_counter() - t_tune
+    progress(f"autotune done: {topts}")
+    solver = brdc.Solver(n, device=dev, **topts)
     stream = torch.cuda.current_stream()
     # one solver (workspace + info word) and one captured graph per matrix:
     # the graph replays the copy of D into the output buffer and every kernel
     graphs = {}
     if not args.eager:
         for g in GENS:
-            graphs[g] = brdc.GraphSolve(brdc.Solver(n, device=dev, leaf_max=args.leaf),
+            graphs[g] = brdc.GraphSolve(brdc.Solver(n, device=dev, **topts),
                                         d_dev[g], e_dev[g])
 
     def step():
@@ -775,6 +790,10 @@ def main():
         "config": {"workload": f"plain BR values-only, N={n}, one solve per generator "
                                f"({', '.join(GENS)}) per step",
                    "n": n, "generators": list(GENS), "leaf_max": solver.opts.leaf_max or 16,
+                   "tuned_options": topts,
+                   "autotune": {"seconds": round(tune_s, 2),
+                                "summed_ms_per_config": {k: round(v, 4)
+                                                         for k, v in tune_ms.items()}},
                    "launch": "eager" if args.eager else "hipGraph replay",
                    "per_generator_ms": {g: round(v, 4) for g, v in per_gen.items()},
                    "per_generator_ms_eager_launch": {g: round(v, 4)

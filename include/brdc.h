@@ -32,11 +32,23 @@ typedef enum brdc_status {
 
 /* Tuning knobs of the device path; zero-initialise for the defaults. */
 typedef struct brdc_options {
-    int leaf_max;     /* largest leaf (rows) of the tree, solved by the leaf
-                         kernel (implicit QL, one lane per leaf), 2..64;
-                         0 selects the default (16) */
-    int reserved[7];  /* must be zero */
+    int leaf_max;           /* largest leaf (rows) of the tree, solved by the leaf
+                               kernel (implicit QL, one lane per leaf), 2..64;
+                               0 selects the default (16) */
+    int stream_min_k;       /* merges with at least this many non-deflated poles
+                               go to the streamed secular kernel, 256..2^28;
+                               0 selects the default (2048) */
+    int secular_lanes_log2; /* log2 of the secular lanes a level aims to keep in
+                               flight (lanes per root follow from it), 12..24;
+                               0 selects the default (18) */
+    int wide_max_waves;     /* widest group (waves) of the product-form and
+                               boundary-row kernels, 1, 2, 4 or 8; 0 selects
+                               the default (8) */
+    int reserved[4];        /* must be zero */
 } brdc_options;
+/* The three launch-shape knobs after leaf_max change which kernel solves a
+ * merge and how sums are split over lanes, so results move within rounding
+ * (not bit for bit) when they change; brdc.autotune picks them by timing. */
 
 /*
  * Host-array entry.  Replaces DSTEDC_DC_BR (cpu/src/dstedc_dc_br.f:7-8), the

@@ -72,10 +72,10 @@ class Solver:
     device tensor ``d`` with the eigenvalues; nothing synchronises.
     """
 
-    def __init__(self, n: int, device="cuda", leaf_max: int = 0):
+    def __init__(self, n: int, device="cuda", leaf_max: int = 0, **tuning):
         import torch
         self.n = int(n)
-        self.opts = options(leaf_max)
+        self.opts = options(leaf_max, **tuning)
         self.bytes = workspace_bytes(self.n, leaf_max)
         self.ws = torch.empty(max(self.bytes, 256), dtype=torch.uint8, device=device)
         self.info = torch.zeros(1, dtype=torch.int32, device=device)
@@ -137,14 +137,14 @@ class BatchSolver:
     synchronise.
     """
 
-    def __init__(self, sizes, device="cuda", leaf_max: int = 0):
+    def __init__(self, sizes, device="cuda", leaf_max: int = 0, **tuning):
         import torch
         L = _lib.load()
         self.sizes = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1))
         self.nmat = int(self.sizes.shape[0])
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes, dtype=np.int64)])
         self.rows = int(self.offsets[-1])
-        self.opts = options(leaf_max)
+        self.opts = options(leaf_max, **tuning)
         self._plan = ctypes.c_void_p()
         _lib.check(L.brdc_batch_plan_create(self.nmat, self.sizes.ctypes.data_as(_lib._ip),
                                             ctypes.byref(self.opts), ctypes.byref(self._plan)),
@@ -244,9 +244,10 @@ def profile_kernels(solver: Solver, problems, stream=None):
     FP64 flops}, "bytes": {class: bytes}} summed over the problems, where the
     flop counts use each solve's own per-merge K and secular evaluation
     counts: 7 flops per pole per evaluation (secular for merges below
-    STREAM_MIN_K poles; secular_stream for the others' evaluations after the
+    the solver's streaming threshold (its stream_min_k option, else
+    STREAM_MIN_K); secular_stream for the others' evaluations after the
     first), 4 per pole for the first, midpoint evaluation of the merges of
-    STREAM_MIN_K poles or more (secular_mid: the sum alone), 3 per pole pair
+    the threshold or more (secular_mid: the sum alone), 3 per pole pair
     (product-form w, non-root merges), 9 per pole pair (boundary rows,
     non-root merges).
     """
@@ -273,7 +274,8 @@ def profile_kernels(solver: Solver, problems, stream=None):
         stats = solver.merge_stats(st)
         k = stats["k"].astype(np.float64)
         ev = stats["evals"].astype(np.float64)
-        big = stats["k"] >= _lib.STREAM_MIN_K
+        kstream = solver.opts.stream_min_k or _lib.STREAM_MIN_K
+        big = stats["k"] >= kstream
         flops["secular"] += float(np.sum(7.0 * k[~big] * ev[~big]))
         # every root of a streamed merge has its first evaluation in secular_mid
         flops["secular_stream"] += float(np.sum(7.0 * k[big] * (ev[big] - k[big])))

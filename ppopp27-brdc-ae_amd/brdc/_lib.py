@@ -41,8 +41,11 @@ STREAM_MIN_K = 2048
 
 
 class Options(ctypes.Structure):
-    """brdc_options: leaf_max (0 = default) plus reserved words."""
-    _fields_ = [("leaf_max", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+    """brdc_options: leaf_max and the launch-shape knobs (0 = default each)
+    plus reserved words."""
+    _fields_ = [("leaf_max", ctypes.c_int), ("stream_min_k", ctypes.c_int),
+                ("secular_lanes_log2", ctypes.c_int), ("wide_max_waves", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -120,9 +123,17 @@ def load() -> ctypes.CDLL:
     return L
 
 
-def options(leaf_max: int = 0) -> Options:
+#: names of the tunable fields of brdc_options besides leaf_max
+TUNING_FIELDS = ("stream_min_k", "secular_lanes_log2", "wide_max_waves")
+
+
+def options(leaf_max: int = 0, stream_min_k: int = 0, secular_lanes_log2: int = 0,
+            wide_max_waves: int = 0) -> Options:
     o = Options()
     o.leaf_max = int(leaf_max)
+    o.stream_min_k = int(stream_min_k)
+    o.secular_lanes_log2 = int(secular_lanes_log2)
+    o.wide_max_waves = int(wide_max_waves)
     return o
 
 

@@ -1,11 +1,25 @@
-"""Autotuner for the device solver's leaf bound.
+"""Autotuner of the device solver's launch shape: the leaf bound and the
+tile/launch knobs of brdc_options.
 
-The leaf bound (rows per leaf, solved by one wavefront's Jacobi sweep in LDS)
-trades leaf work (O(m^3) per leaf, a few waves per CU) against tree depth
-(one launch sequence per level).  The best value depends on the size mix and
-the spectrum, so it is measured: each candidate is timed as a captured-graph
-replay of the actual workload, and the fastest wins.  Results can be cached
-in a JSON file keyed by the device name and a signature of the size list.
+* leaf_max: rows per leaf (solved by the leaf kernel, implicit QL with one
+  lane per leaf, O(m^2) per leaf) against tree depth (one launch sequence per
+  level);
+* stream_min_k: the merge size from which the streamed secular kernel (a
+  persistent grid sharing LDS-staged pole chunks) takes over from the
+  lanes-per-root kernel;
+* secular_lanes_log2: the secular lanes a level aims to keep in flight, from
+  which the lanes per root follow (more lanes: shorter passes, more waves);
+* wide_max_waves: the widest group of the product-form and boundary-row
+  kernels (waves splitting one 64-item group's stream).
+
+The best values depend on the size mix and the spectrum, so they are
+measured: each candidate is timed as a captured-graph replay of the actual
+workload (all its matrices), one knob at a time in the order above, each
+keeping the best value found so far for the others (a coordinate search; the
+defaults start it).  Results can be cached in a JSON file keyed by the device
+name and a signature of the sizes.  The knobs other than leaf_max move results
+within rounding only (they change which kernel solves a merge and how its
+sums are split).
 """
 from __future__ import annotations
 
@@ -18,6 +32,12 @@ import numpy as np
 from . import BatchSolver, GraphSolve, Solver
 
 CANDIDATES = (8, 12, 16, 24, 32, 48, 64)
+#: candidate values of each launch-shape knob, the default first
+KNOB_CANDIDATES = {
+    "stream_min_k": (2048, 1024, 4096),
+    "secular_lanes_log2": (18, 17, 19),
+    "wide_max_waves": (8, 4, 2),
+}
 
 
 def _signature(kind: str, sizes, device_name: str) -> str:
@@ -55,44 +75,85 @@ def _store(path, cache):
         os.replace(tmp, path)
 
 
+def _search(time_of, leaf_candidates, knobs: bool):
+    """Coordinate search: time_of(config) -> ms for a dict of options."""
+    timings = {}
+
+    def run(cfg):
+        key = ",".join(f"{k}={cfg[k]}" for k in sorted(cfg))
+        if key not in timings:
+            timings[key] = time_of(cfg)
+        return timings[key]
+
+    best = {"leaf_max": int(leaf_candidates[0])}
+    if knobs:
+        best.update({k: v[0] for k, v in KNOB_CANDIDATES.items()})
+    for leaf in leaf_candidates:
+        cfg = dict(best, leaf_max=int(leaf))
+        if run(cfg) < run(best):
+            best = cfg
+    if knobs:
+        for name, values in KNOB_CANDIDATES.items():
+            for v in values:
+                cfg = dict(best, **{name: int(v)})
+                if run(cfg) < run(best):
+                    best = cfg
+    return best, timings
+
+
 def tune_batched(d_cat, e_cat, sizes, candidates=CANDIDATES, reps: int = 5,
-                 cache_path: str | None = None) -> dict:
-    """Pick the leaf bound for a batch.  d_cat, e_cat: the batch's
+                 cache_path: str | None = None, knobs: bool = True) -> dict:
+    """Pick the launch shape for a batch.  d_cat, e_cat: the batch's
     concatenated rows as float64 device tensors (read only).  Returns
-    {"leaf_max": best, "ms": {leaf: ms per batch}, "cached": bool}."""
+    {"leaf_max": best leaf, "options": {every tuned option}, "ms": {config:
+    ms per batch}, "cached": bool}."""
     import torch
     dev = d_cat.device
-    key = _signature("batch", sizes, torch.cuda.get_device_name(dev))
+    key = _signature("batch" + ("+knobs" if knobs else ""), sizes,
+                     torch.cuda.get_device_name(dev))
     cache = _load(cache_path)
     if key in cache:
         return dict(cache[key], cached=True)
-    timings = {}
-    for leaf in candidates:
-        gs = GraphSolve(BatchSolver(sizes, device=dev, leaf_max=leaf), d_cat, e_cat)
-        timings[int(leaf)] = _time_graph(gs, reps)
+
+    def time_of(cfg):
+        gs = GraphSolve(BatchSolver(sizes, device=dev, **cfg), d_cat, e_cat)
+        ms = _time_graph(gs, reps)
         del gs
-    best = min(timings, key=timings.get)
-    res = {"leaf_max": int(best), "ms": timings}
+        return ms
+
+    best, timings = _search(time_of, candidates, knobs)
+    res = {"leaf_max": int(best["leaf_max"]), "options": best, "ms": timings}
     cache[key] = res
     _store(cache_path, cache)
     return dict(res, cached=False)
 
 
-def tune_single(d, e, candidates=CANDIDATES, reps: int = 5, cache_path: str | None = None) -> dict:
-    """Pick the leaf bound for one matrix (d, e: float64 device tensors)."""
+def tune_single(problems, candidates=(12, 16, 24, 32), reps: int = 5,
+                cache_path: str | None = None, knobs: bool = True) -> dict:
+    """Pick the launch shape for single solves of one order n: problems is a
+    list of (d, e) float64 device tensors of equal length (for instance one
+    matrix of each kind the caller solves); the summed graph-replay time of
+    all of them is minimised.  Returns {"options": {...}, "ms": {config: summed
+    ms}, "cached": bool}."""
     import torch
-    n = d.numel()
-    key = _signature("single", [n], torch.cuda.get_device_name(d.device))
+    n = problems[0][0].numel()
+    dev = problems[0][0].device
+    key = _signature("single" + ("+knobs" if knobs else ""), [n, len(problems)],
+                     torch.cuda.get_device_name(dev))
     cache = _load(cache_path)
     if key in cache:
         return dict(cache[key], cached=True)
-    timings = {}
-    for leaf in candidates:
-        gs = GraphSolve(Solver(n, device=d.device, leaf_max=leaf), d, e)
-        timings[int(leaf)] = _time_graph(gs, reps)
-        del gs
-    best = min(timings, key=timings.get)
-    res = {"leaf_max": int(best), "ms": timings}
+
+    def time_of(cfg):
+        total = 0.0
+        for d, e in problems:
+            gs = GraphSolve(Solver(n, device=dev, **cfg), d, e)
+            total += _time_graph(gs, reps)
+            del gs
+        return total
+
+    best, timings = _search(time_of, candidates, knobs)
+    res = {"options": best, "ms": timings}
     cache[key] = res
     _store(cache_path, cache)
     return dict(res, cached=False)

@@ -41,6 +41,15 @@ constexpr int kPackMinMerges = 4096;
 constexpr int kSecMaxWaves = 3072;  // secular grid cap: about 3 resident waves per SIMD
 constexpr int kStreamWaves = 8;     // waves per workgroup of the streamed secular kernel
 constexpr int kStreamMinK = 2048;   // merges with at least this many poles are streamed
+
+// Launch-shape knobs (brdc_options; 0 there selects these defaults): the
+// streaming threshold, the secular lanes in flight, and the widest group of
+// the product-form and boundary-row kernels.
+struct Tuning {
+    int stream_min_k = kStreamMinK;
+    int target_lanes = kTargetLanes;
+    int wide_waves = kMaxWaves;
+};
 constexpr int kMidMaxGroups = 1024; // midpoint-pass grid cap (tiles of 64 roots)
 
 // Workgroups of the streamed secular kernel that fit the device at once
@@ -155,9 +164,24 @@ int leaf_of(const brdc_options* o) {
     return o->leaf_max;
 }
 
+Tuning tuning_of(const brdc_options* o) {
+    Tuning t;
+    if (!o) return t;
+    if (o->stream_min_k) t.stream_min_k = o->stream_min_k;
+    if (o->secular_lanes_log2) t.target_lanes = 1 << o->secular_lanes_log2;
+    if (o->wide_max_waves) t.wide_waves = o->wide_max_waves;
+    return t;
+}
+
 bool options_ok(const brdc_options* o) {
     if (!o) return true;
     if (o->leaf_max != 0 && (o->leaf_max < 2 || o->leaf_max > kMaxLeaf)) return false;
+    if (o->stream_min_k != 0 && (o->stream_min_k < 256 || o->stream_min_k > (1 << 28)))
+        return false;
+    if (o->secular_lanes_log2 != 0 && (o->secular_lanes_log2 < 12 || o->secular_lanes_log2 > 24))
+        return false;
+    const int w = o->wide_max_waves;
+    if (w != 0 && w != 1 && w != 2 && w != 4 && w != 8) return false;
     for (int r : o->reserved)
         if (r != 0) return false;
     return true;
@@ -249,15 +273,17 @@ struct LevelRun {
 // A whole solve: rows, matrices, leaves and levels (deepest first).
 struct Schedule {
     int rows = 0, nmat = 1;
+    Tuning tune{};
     const int* moff = nullptr;  // device: first row of each matrix (batch), else nullptr
     Geo leaves{};
     int nleaves = 0, leaf_rows = 0;
     std::vector<LevelRun> levels;
 };
 
-Schedule tree_schedule(int n, int leaf) {
+Schedule tree_schedule(int n, int leaf, const Tuning& tune) {
     Schedule S;
     S.rows = n;
+    S.tune = tune;
     const int levels = tree_levels(n, leaf);
     S.leaves = Geo{n, levels, nullptr};
     S.nleaves = 1 << levels;
@@ -357,7 +383,7 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
         const dim3 grid(merges * bpm);
         // the O(K^2) kernels: one block per 64 roots/poles, WL waves wide
         const int bpg = (maxsz + kGroup - 1) / kGroup;
-        const int WL = waves_for(maxsz, kMaxWaves);
+        const int WL = waves_for(maxsz, S.tune.wide_waves);
         int* counts = counts_all + kCountStride * lv.counts_rec;
         const int nxt = cur ^ 1;
 
@@ -392,33 +418,34 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
                            aq, gd, gq1, gq2, td, tq1, tq2, counts);
         }
         BRDC_LAUNCH_CHECK();
-        // merges of K >= kStreamMinK poles go to the streamed kernel, the
+        // merges of K >= kstream poles go to the streamed kernel, the
         // rest to the lanes-per-root kernel
-        const bool streamed = (maxsz >= kStreamMinK);
+        const int kstream = S.tune.stream_min_k, target = S.tune.target_lanes;
+        const bool streamed = (maxsz >= kstream);
         if (maxsz <= kPackMaxSize && merges >= kPackMinMerges) {
             // many small merges: roots packed across merges (rend is free
             // between deflations and holds the root offsets)
             ProfScope ps(kSecular, st);
             hipLaunchKernelGGL(k_root_scan, dim3(1), dim3(kScanThreads), 0, st, merges,
-                               streamed ? kStreamMinK : INT_MAX, (const int*)counts, rend);
+                               streamed ? kstream : INT_MAX, (const int*)counts, rend);
             BRDC_LAUNCH_CHECK();
             const int blocks = kSecMaxWaves / kSecWaves;
             hipLaunchKernelGGL(k_secular_packed<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves),
-                               0, st, g, merges, (const int*)rend, (const double*)ew,
+                               0, st, g, merges, target, (const int*)rend, (const double*)ew,
                                (const int*)counts, (const double*)ad, (const double*)az,
                                (const double2*)adz, rlam, rs, rn1, rn2, counts, S.moff, info);
         } else {
             ProfScope ps(kSecular, st);
             // wave slots per merge: the most any K it handles can need; the
             // grid is capped and walks (merge, slot) tasks
-            const int slots = secular_slots(std::min(maxsz, kStreamMinK - 1), merges);
+            const int slots = secular_slots(std::min(maxsz, kstream - 1), merges, target);
             const int waves = std::min(merges * slots, kSecMaxWaves);
             const int blocks = (waves + kSecWaves - 1) / kSecWaves;
             // LDS pole stage per wave: the level's largest merge, capped
             const int stage_k = std::min(maxsz, kSecStageK);
             const size_t lds = sizeof(double2) * kSecWaves * stage_k;
             hipLaunchKernelGGL(k_secular<kSecWaves>, dim3(blocks), dim3(kWave * kSecWaves), lds,
-                               st, g, merges, slots, streamed ? kStreamMinK : INT_MAX, stage_k,
+                               st, g, merges, slots, streamed ? kstream : INT_MAX, stage_k, target,
                                (const double*)ew, (const int*)counts, (const double*)ad,
                                (const double*)az, (const double2*)adz, rlam, rs, rn1, rn2, counts,
                                S.moff, info);
@@ -429,7 +456,7 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
                 ProfScope ps(kSecularMid, st);
                 hipLaunchKernelGGL(k_secular_mid<kStreamWaves>,
                                    dim3(std::min(kMidMaxGroups, (n + kWave - 1) / kWave)),
-                                   dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
+                                   dim3(kWave * kStreamWaves), 0, st, g, merges, kstream,
                                    (const double*)ew, (const int*)counts, (const double2*)adz,
                                    rlam);
             }
@@ -437,7 +464,7 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
             ProfScope ps(kSecularStream, st);
             const int groups = std::min(stream_groups(), (n + kWave - 1) / kWave);
             hipLaunchKernelGGL(k_secular_stream<kStreamWaves>, dim3(groups),
-                               dim3(kWave * kStreamWaves), 0, st, g, merges, kStreamMinK,
+                               dim3(kWave * kStreamWaves), 0, st, g, merges, kstream,
                                (const double*)ew, (const int*)counts, (const double2*)adz, rlam,
                                rs, rn1, rn2, counts, S.moff, info);
             BRDC_LAUNCH_CHECK();
@@ -473,8 +500,8 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
 }
 
 int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, hipStream_t st,
-                  int* info_user, int leaf) {
-    return enqueue(tree_schedule(n, leaf), d, e, ws, L, st, at<int>(ws, L.info), info_user);
+                  int* info_user, int leaf, const Tuning& tune) {
+    return enqueue(tree_schedule(n, leaf, tune), d, e, ws, L, st, at<int>(ws, L.info), info_user);
 }
 
 // ---------------------------------------------------------------------------
@@ -493,6 +520,7 @@ struct HostLevel {
 
 struct brdc_batch_plan {
     int nmat = 0, rows = 0, leaf = 0;
+    brdc::Tuning tune{};
     std::vector<int> moff;           // nmat + 1
     std::vector<int> tprefix;        // gradient tasks (64 eigenvalues) before each matrix
     std::vector<long long> poff;     // gradient partial-sum offset of each matrix
@@ -625,6 +653,7 @@ Schedule batch_schedule(const brdc_batch_plan& p, void* ws) {
     Schedule S;
     S.rows = p.rows;
     S.nmat = p.nmat;
+    S.tune = p.tune;
     S.moff = at<int>(ws, p.layout.moff);
     const int4* plan = at<int4>(ws, p.layout.plan);
     S.leaves = Geo{p.rows, -1, plan};
@@ -685,7 +714,7 @@ int brdc_dstebr_device_ex(int n, double* d_dev, const double* e_dev, void* works
     if (workspace_bytes < L.total || (reinterpret_cast<uintptr_t>(workspace) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
     return enqueue_solve(n, d_dev, e_dev, workspace, L, static_cast<hipStream_t>(stream), info_dev,
-                         leaf);
+                         leaf, tuning_of(opts));
 }
 
 int brdc_dstebr_device(int n, double* d_dev, const double* e_dev, void* workspace,
@@ -825,7 +854,7 @@ int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork
                hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice) != hipSuccess) {
         result = -1000 - BRDC_STATUS_HIP_ERROR;
     } else {
-        const int st = enqueue_solve(n, dd, de, ws, L, nullptr, dinfo, kDefaultLeaf);
+        const int st = enqueue_solve(n, dd, de, ws, L, nullptr, dinfo, kDefaultLeaf, Tuning{});
         if (st != BRDC_STATUS_SUCCESS ||
             hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
@@ -874,6 +903,7 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
     p->nmat = nmat;
     p->rows = (int)rows;
     p->leaf = leaf_of(opts);
+    p->tune = tuning_of(opts);
     p->moff.resize((size_t)nmat + 1);
     std::vector<int> lev((size_t)nmat, 0);
     int H = 0;
@@ -1174,5 +1204,5 @@ extern "C" int brdc_dstebr_device_noalloc(int n, double* d_dev, const double* e_
     const int st = scratch_address(&ws);
     if (st != BRDC_STATUS_SUCCESS) return st;
     return enqueue_solve(n, d_dev, e_dev, ws, L, static_cast<hipStream_t>(stream), info_dev,
-                         leaf);
+                         leaf, tuning_of(opts));
 }

@@ -1160,10 +1160,10 @@ __device__ __forceinline__ void add_secular_counts(int* __restrict__ rec, int it
 // than the few spills it costs.
 template <int WL>
 __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_secular(
-    Geo g, int merges, int slots, int kstream, int stage_k, const double* __restrict__ ew,
-    const int* counts, const double* __restrict__ ad, const double* __restrict__ az,
-    const double2* __restrict__ adz, double* __restrict__ rlam, double2* __restrict__ rs,
-    double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw,
+    Geo g, int merges, int slots, int kstream, int stage_k, int target,
+    const double* __restrict__ ew, const int* counts, const double* __restrict__ ad,
+    const double* __restrict__ az, const double2* __restrict__ adz, double* __restrict__ rlam,
+    double2* __restrict__ rs, double* __restrict__ rn1, double* __restrict__ rn2, int* counts_rw,
     const int* __restrict__ moff, int* info) {
     extern __shared__ double2 sec_stage[];
     const int ntask = merges * slots;
@@ -1177,7 +1177,7 @@ __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3
         // evenly over the waves whatever the slot over-provisioning
         const int slot = task / merges, merge = task - slot * merges;
         const int K = uniform(counts[kCountStride * merge]);
-        const int L = secular_lanes(K, merges);
+        const int L = secular_lanes(K, merges, target);
         const int rpw = kWave / L;  // roots per wave
         if (slot * rpw >= K || K >= kstream) continue;  // idle slot / streamed merge
         const Item it = g.item(merge);
@@ -1267,14 +1267,14 @@ __global__ __launch_bounds__(kScanThreads) void k_root_scan(int merges, int kstr
 // lone root at L lanes, whatever it runs beside.
 template <int WL>
 __global__ __launch_bounds__(kWave * WL) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_secular_packed(
-    Geo g, int merges, const int* __restrict__ roff, const double* __restrict__ ew,
+    Geo g, int merges, int target, const int* __restrict__ roff, const double* __restrict__ ew,
     const int* __restrict__ counts, const double* __restrict__ ad,
     const double* __restrict__ az, const double2* __restrict__ adz, double* __restrict__ rlam,
     double2* __restrict__ rs, double* __restrict__ rn1, double* __restrict__ rn2,
     int* counts_rw, const int* __restrict__ moff, int* info) {
     const int total = uniform(roff[merges]);
     if (total == 0) return;
-    const int L = secular_lanes((total + merges - 1) / merges, merges);
+    const int L = secular_lanes((total + merges - 1) / merges, merges, target);
     const int rpg = kWave / L;  // root positions per wave
     const long long groups = (total + rpg - 1) / rpg;
     const int nwave = gridDim.x * WL;

@@ -12,17 +12,18 @@ constexpr int kWave = 64;
 constexpr int kCountStride = 8;
 
 // lanes per root: about 4+ poles per lane per pass, and no more lanes than
-// the level needs to fill the machine (~kTargetLanes lanes in flight)
+// the level needs to fill the machine (~target lanes in flight; kTargetLanes
+// unless tuned)
 constexpr int kTargetLanes = 1 << 18;
-__host__ __device__ inline int secular_lanes(int K, int merges) {
+__host__ __device__ inline int secular_lanes(int K, int merges, int target = kTargetLanes) {
     int L = 1;
-    while (L < kWave && 4 * 2 * L <= K && (long long)merges * K * (2 * L) <= kTargetLanes) L <<= 1;
+    while (L < kWave && 4 * 2 * L <= K && (long long)merges * K * (2 * L) <= target) L <<= 1;
     return L;
 }
 
 // waves a merge of K poles needs at L lanes per root
-__host__ __device__ inline int secular_waves(int K, int merges) {
-    const int L = secular_lanes(K, merges);
+__host__ __device__ inline int secular_waves(int K, int merges, int target = kTargetLanes) {
+    const int L = secular_lanes(K, merges, target);
     return (K * L + kWave - 1) / kWave;
 }
 
@@ -30,14 +31,14 @@ __host__ __device__ inline int secular_waves(int K, int merges) {
 // monotone in K (a smaller K can be given more lanes per root), but it grows
 // with K while the lane count holds, so its maximum sits at the upper end of
 // a constant-lane run: K = maxsz, K = 8L - 1 (the next K doubles L) or
-// K = kTargetLanes / (merges * L) (the next K halves L).
-inline int secular_slots(int maxsz, int merges) {
-    int best = secular_waves(maxsz, merges);
+// K = target / (merges * L) (the next K halves L).
+inline int secular_slots(int maxsz, int merges, int target = kTargetLanes) {
+    int best = secular_waves(maxsz, merges, target);
     for (int L = 1; L <= kWave; L <<= 1) {
-        const long long cand[2] = {8LL * L - 1, kTargetLanes / ((long long)merges * L)};
+        const long long cand[2] = {8LL * L - 1, target / ((long long)merges * L)};
         for (long long k : cand)
             if (k >= 1 && k <= maxsz) {
-                const int w = secular_waves((int)k, merges);
+                const int w = secular_waves((int)k, merges, target);
                 if (w > best) best = w;
             }
     }

@@ -228,3 +228,27 @@ def test_fortran_caller_links_and_runs(tmp_path, route):
     assert got["smalllwork"] == ["-5"]
     assert got["smallliwork"] == ["-7"]
     assert got["n1"] == ["0", "3.500"]
+
+
+def test_tuning_options_are_validated_without_a_gpu():
+    """brdc_options' launch-shape knobs: 0 keeps the default, in-range values
+    are accepted, out-of-range ones rejected with BRDC_STATUS_INVALID_VALUE
+    (checked by the workspace query, which touches no device)."""
+    L = _lib.load()
+    out = ctypes.c_size_t(0)
+
+    def status(**kw):
+        o = _lib.options(**kw)
+        return L.brdc_dstebr_device_workspace_ex(4096, ctypes.byref(o), ctypes.byref(out))
+    assert status() == 0
+    for kw in ({"stream_min_k": 256}, {"stream_min_k": 1 << 20}, {"secular_lanes_log2": 12},
+               {"secular_lanes_log2": 24}, {"wide_max_waves": 1}, {"wide_max_waves": 4},
+               {"leaf_max": 24, "stream_min_k": 1024, "secular_lanes_log2": 17,
+                "wide_max_waves": 2}):
+        assert status(**kw) == 0, kw
+    for kw in ({"stream_min_k": 100}, {"stream_min_k": -1}, {"secular_lanes_log2": 11},
+               {"secular_lanes_log2": 25}, {"wide_max_waves": 3}, {"wide_max_waves": 16}):
+        assert status(**kw) == 3, kw
+    o = _lib.options()
+    o.reserved[2] = 1
+    assert L.brdc_dstebr_device_workspace_ex(4096, ctypes.byref(o), ctypes.byref(out)) == 3

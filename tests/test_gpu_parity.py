@@ -277,3 +277,41 @@ def test_harness_gpu_columns(gpu):
         assert np.all(st["evals"][big] >= st["k"][big])
         assert set(r["level_ms"]) == set(range(-1, int(st["level"].max()) + 1))
     assert "iter_max" in out.getvalue()
+
+
+@pytest.mark.parametrize("knobs", [{"stream_min_k": 1024}, {"stream_min_k": 4096},
+                                   {"secular_lanes_log2": 16}, {"secular_lanes_log2": 20},
+                                   {"wide_max_waves": 1}, {"wide_max_waves": 2},
+                                   {"leaf_max": 32, "stream_min_k": 512, "secular_lanes_log2": 17,
+                                    "wide_max_waves": 4}])
+def test_launch_shape_knobs_keep_parity(gpu, orc, knobs):
+    """Every launch-shape knob the autotuner sets keeps the results within the
+    bar: an evenly spread diagonal with a constant coupling (n = 6000, top
+    merges K >= 2048), Toeplitz n = 8194 (closed form) and a batch."""
+    import brdc
+    torch = gpu
+    dev = torch.device("cuda:0")
+    cases = [(np.linspace(-1.0, 1.0, 6000), np.full(5999, 0.3), None),
+             (np.full(8194, 2.0), np.full(8193, 1.0),
+              np.sort(2.0 + 2.0 * np.cos(np.arange(1, 8195) * np.pi / 8195)))]
+    for d, e, ref in cases:
+        n = d.shape[0]
+        if ref is None:
+            ref, info_ref = orc.dstebr(d, e)
+            assert info_ref == 0
+        s = brdc.Solver(n, device=dev, **knobs)
+        w = torch.from_numpy(d).to(dev)
+        s.solve(w, torch.from_numpy(e).to(dev))
+        torch.cuda.synchronize()
+        assert int(s.info.item()) == 0 and rel_err(w.cpu().numpy(), ref) <= TOL
+    from brdc.generators import pack_batch
+    mats = [generate("normal", m, seed_for(7, m, b)) for b, m in enumerate((300, 2500, 4100))]
+    d_cat, e_cat, off = pack_batch(mats)
+    bs = brdc.BatchSolver([300, 2500, 4100], device=dev, **knobs)
+    w = torch.from_numpy(d_cat).to(dev)
+    bs.solve(w, torch.from_numpy(e_cat).to(dev))
+    torch.cuda.synchronize()
+    ws = bs.split(w.cpu().numpy())
+    for b, (d, e) in enumerate(mats):
+        ref, _ = orc.dstebr(d, e)
+        assert rel_err(ws[b], ref) <= TOL
