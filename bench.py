@@ -13,7 +13,7 @@ GPU's error there as max |dw| / max(1, max |w|).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 49152]
        [--no-cpu-baseline] [--leaf L] [--no-tune] [--eager] [--no-batched] [--no-grad]
-       [--no-vendor] [--no-compare]
+       [--no-sizes] [--no-vendor] [--no-compare]
        [--only batched|differentiated|vendor|config3|sizes]
 
 Besides the main line's fields (the value, a roofline record for the
@@ -581,6 +581,8 @@ def main():
                     help="skip the batched-workload section")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the forward+backward section")
+    ap.add_argument("--no-sizes", action="store_true",
+                    help="skip the N=16384/32768 and shipped-input section")
     ap.add_argument("--no-vendor", action="store_true",
                     help="skip the rocSOLVER comparison (about 20 s)")
     ap.add_argument("--no-compare", action="store_true",
@@ -809,8 +811,9 @@ def main():
     }
     if cpu_rec:
         line["speedup_vs_cpu_baseline"] = round(cpu_rec["value"] / value, 2) if value > 0 else None
-    progress("other sizes and the shipped input start")
-    line["other_sizes_and_shipped_input"] = bench_other_sizes_and_shipped_input()
+    if not args.no_sizes:
+        progress("other sizes and the shipped input start")
+        line["other_sizes_and_shipped_input"] = bench_other_sizes_and_shipped_input()
     if not args.no_batched:
         progress("batched section starts")
         line["batched"] = bench_batched(max(3, args.steps // 2), max(1, args.warmup),

@@ -725,9 +725,19 @@ __global__ void k_grad_mask(int rows, int nmat, const int* __restrict__ moff,
     const bool member = (k > 0 && W[k] - W[k - 1] <= tol) || (k + 1 < n && W[k + 1] - W[k] <= tol);
     gm[j] = member ? 0.0 : g[j];
     if ((k & (kWave - 1)) == 0) {
-        bool any = false;
-        for (int q = k; q < min(k + kWave, n); q++) any = any || cluster_starts(W, n, q, tol);
-        tflag[t0 + k / kWave] = any ? 1 : 0;
+        // the task's 64 eigenvalues, walked with a sliding window and no
+        // early exit, so the loads of the unrolled steps are in flight together
+        const int q1 = min(k + kWave, n);
+        int any = 0;
+        double prev = (k > 0) ? W[k - 1] : 0.0, cur = W[k];
+#pragma unroll 8
+        for (int q = k; q < q1; q++) {
+            const double next = (q + 1 < n) ? W[q + 1] : 0.0;
+            any |= (int)((q == 0 || cur - prev > tol) && (q + 1 < n && next - cur <= tol));
+            prev = cur;
+            cur = next;
+        }
+        tflag[t0 + k / kWave] = any;
     }
 }
 
