@@ -235,7 +235,13 @@ int brdc_batch_plan_stats(const brdc_batch_plan *plan, int *out, int nout);
  * reproducible.  Asynchronous on stream; returns a status for the launch.
  * For n >= 8192 the eigenvectors are crossed segment by segment with stored
  * transfers; a transfer whose partial products exceed 2^400 is replaced by a
- * walk of the rows.  The environment variable BRDC_GRAD_SCAN_LIMIT (read at
+ * walk of the rows.  The scratch grows as about n^2 / 8 doubles (1.3 GB at
+ * n = 16384, 18.9 GB at n = 65536); above 2^37 bytes (n of about 170000) the
+ * workspace queries and the entries return BRDC_STATUS_INVALID_VALUE.
+ * Eigenvalues within 2^-44 ||T|| of a neighbour (numerically repeated: for
+ * instance a matrix split into identical blocks) get their vectors from
+ * inverse iteration with Gram-Schmidt inside the cluster, so the cluster's
+ * vectors stay orthonormal.  The environment variable BRDC_GRAD_SCAN_LIMIT (read at
  * each call, a value in [0, 2^400]) lowers that bound; 0 walks every row.
  */
 int brdc_dstebr_grad_workspace(int n, size_t *bytes);

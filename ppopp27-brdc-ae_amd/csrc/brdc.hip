@@ -548,6 +548,11 @@ struct GradShape {
 // checkpoint doubles of one task of an n-row matrix
 inline long long grad_slot(int n) { return grad_cp_slot(n); }
 
+// The backward's scratch grows as n^2 / 8 doubles (checkpoints of every
+// task's pivot chains); above this many bytes (128 GiB, about n = 170000 for
+// one matrix) the queries and entries refuse with BRDC_STATUS_INVALID_VALUE.
+constexpr size_t kGradScratchMax = (size_t)1 << 37;
+
 // rows: all rows of the call; nmax: its largest matrix (the cluster kernel's
 // per-wave arrays)
 GradShape grad_shape(int ntask, long long cp_doubles, long long partial_doubles, int ngamma,
@@ -1102,7 +1107,7 @@ int brdc_dstebr_grad_workspace(int n, size_t* bytes) {
     grad_counts_one(n, count);
     *bytes = grad_shape(tasks, tasks * grad_slot(n), (long long)tasks * 2 * n, count[1], n, n)
                  .total;
-    return BRDC_STATUS_SUCCESS;
+    return (*bytes > kGradScratchMax) ? BRDC_STATUS_INVALID_VALUE : BRDC_STATUS_SUCCESS;
 }
 
 int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
@@ -1116,6 +1121,8 @@ int brdc_dstebr_grad_device(int n, const double* d_dev, const double* e_dev,
     const long long partial = (long long)tasks * 2 * n, cpd = tasks * grad_slot(n);
     int count[4];
     grad_counts_one(n, count);
+    if (grad_shape(tasks, cpd, partial, count[1], n, n).total > kGradScratchMax)
+        return BRDC_STATUS_INVALID_VALUE;
     if (scratch_bytes < grad_shape(tasks, cpd, partial, count[1], n, n).total ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
@@ -1128,7 +1135,7 @@ int brdc_batch_grad_workspace(const brdc_batch_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return BRDC_STATUS_INVALID_POINTER;
     *bytes = grad_shape(plan->tprefix.back(), plan->cpoff.back(), plan->poff.back(),
                         plan->gcount[1], plan->rows, plan->rows_max).total;
-    return BRDC_STATUS_SUCCESS;
+    return (*bytes > kGradScratchMax) ? BRDC_STATUS_INVALID_VALUE : BRDC_STATUS_SUCCESS;
 }
 
 int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const double* e_dev,
@@ -1142,9 +1149,10 @@ int brdc_batch_grad(const brdc_batch_plan* plan, const double* d_dev, const doub
     const Layout& L = plan->layout;
     const int ntask = plan->tprefix.back();
     const long long partial = plan->poff.back();
-    if (workspace_bytes < L.total ||
-        scratch_bytes < grad_shape(ntask, plan->cpoff.back(), partial, plan->gcount[1],
-                                   plan->rows, plan->rows_max).total ||
+    const size_t need = grad_shape(ntask, plan->cpoff.back(), partial, plan->gcount[1],
+                                   plan->rows, plan->rows_max).total;
+    if (need > kGradScratchMax) return BRDC_STATUS_INVALID_VALUE;
+    if (workspace_bytes < L.total || scratch_bytes < need ||
         (reinterpret_cast<uintptr_t>(scratch) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
     void* ws = const_cast<void*>(workspace);

@@ -252,3 +252,20 @@ def test_tuning_options_are_validated_without_a_gpu():
     o = _lib.options()
     o.reserved[2] = 1
     assert L.brdc_dstebr_device_workspace_ex(4096, ctypes.byref(o), ctypes.byref(out)) == 3
+
+
+def test_grad_scratch_query_and_its_bound():
+    """The backward's scratch (checkpointed pivot chains of every task) grows
+    as n^2: about 17.6 GiB at n = 65536, the largest configured order; orders
+    whose scratch would pass 2^37 bytes are refused with
+    BRDC_STATUS_INVALID_VALUE before any device work."""
+    L = _lib.load()
+    out = ctypes.c_size_t(0)
+    assert L.brdc_dstebr_grad_workspace(65536, ctypes.byref(out)) == 0
+    assert 16 * 2 ** 30 < out.value < 20 * 2 ** 30
+    small = ctypes.c_size_t(0)
+    assert L.brdc_dstebr_grad_workspace(16384, ctypes.byref(small)) == 0
+    assert 10 < out.value / small.value < 20  # quadratic growth
+    assert L.brdc_dstebr_grad_workspace(200000, ctypes.byref(out)) == 3
+    assert L.brdc_dstebr_grad_device(200000, None, None, None, None, None, None, None, 0,
+                                     None) == 2  # pointers are checked first
