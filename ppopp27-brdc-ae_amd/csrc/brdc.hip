@@ -616,8 +616,8 @@ int enqueue_grad(int rows, int nmat, int nmax, const int* moff, const int* tpref
     // members of numerically degenerate clusters leave the twisted
     // factorization (weight 0 there) for the cluster kernel
     int* tflag = at<int>(scratch, G.tflag);
-    hipLaunchKernelGGL(k_grad_mask, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat, moff,
-                       tprefix, w, g, gm, tflag);
+    hipLaunchKernelGGL(k_grad_mask, dim3((ntask + 3) / 4), dim3(4 * kWave), 0, st, ntask, nmat,
+                       moff, tprefix, n_one, w, g, gm, tflag);
     BRDC_LAUNCH_CHECK();
     size_t base = 0;
     for (int l = 0; l < 4; l++) {
@@ -638,8 +638,9 @@ int enqueue_grad(int rows, int nmat, int nmax, const int* moff, const int* tpref
                        st, ntask, nmat, moff, tprefix, poff, n_one, nmax, d, e, w, g,
                        (const int*)tflag, at<double>(scratch, G.clus), part);
     BRDC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_grad_reduce, dim3((rows + 255) / 256), dim3(256), 0, st, rows, nmat,
-                       moff, tprefix, poff, n_one, (const double*)part, dd, de);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((rows + kReduceRows - 1) / kReduceRows),
+                       dim3(kReduceRows * kReduceSlices), 0, st, rows, nmat, moff, tprefix, poff,
+                       n_one, (const double*)part, dd, de);
     BRDC_LAUNCH_CHECK();
     return BRDC_STATUS_SUCCESS;
 }

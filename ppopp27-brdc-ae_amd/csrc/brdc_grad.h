@@ -640,14 +640,17 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 else x = (k <= r) ? 1.0 : zlo;
                 M[j] = (k < r) ? P[j] : x;  // z of row lo+j
             }
-            // v[j] = dD contribution of row lo+j, v[16+j] = dE contribution
+            // v[j] = dD contribution of row lo+j, v[16+j] = half the dE
+            // contribution (the factor 2 is applied to the row total: scaling
+            // by 2 is exact, so the sums are those of the doubled terms)
             double v[2 * kSeg];
 #pragma unroll
             for (int j = 0; j < kSeg; j++) {
                 const double zn = (j + 1 < kSeg) ? M[j + 1] : zhi;
                 const double znext = (j + 1 < len) ? zn : ((j + 1 == len) ? zhi : 0.0);
-                v[j] = sc * M[j] * M[j];
-                v[kSeg + j] = 2.0 * sc * M[j] * znext;
+                const double t = sc * M[j];
+                v[j] = t * M[j];
+                v[kSeg + j] = t * znext;
             }
             // reduce-scatter over the wave: afterwards lane L holds the total
             // of value L / kShare
@@ -657,7 +660,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             if ((lane & (kShare - 1)) == 0) {
                 const int idx = lane / kShare;
                 const int row = idx & (kSeg - 1);
-                if (row < len) part[(idx < kSeg ? 0 : n) + lo + row] = total;
+                if (row < len) part[(idx < kSeg ? 0 : n) + lo + row] = (idx < kSeg) ? total : 2.0 * total;
             }
         }
     }
@@ -698,47 +701,34 @@ __device__ __forceinline__ bool cluster_starts(const double* __restrict__ w, int
 }
 
 // gm[j] = g[j], or 0 for a member of a cluster (rows numbered as in the
-// batch layout; moff == nullptr: one matrix of rows rows); tflag[t] = 1 when
-// a cluster starts among task t's 64 eigenvalues (written by the thread of
-// the task's first row)
-__global__ void k_grad_mask(int rows, int nmat, const int* __restrict__ moff,
-                            const int* __restrict__ tprefix, const double* __restrict__ w,
-                            const double* __restrict__ g, double* __restrict__ gm,
-                            int* __restrict__ tflag) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= rows) return;
-    int off = 0, n = rows, t0 = 0;
-    if (moff) {
-        int lo = 0, hi = nmat;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (moff[mid] <= j) lo = mid;
-            else hi = mid;
-        }
-        off = moff[lo];
-        n = moff[lo + 1] - off;
-        t0 = tprefix[lo];
-    }
-    const int k = j - off;
+// batch layout; moff == nullptr: one matrix of n_one rows); tflag[t] = 1 when
+// a cluster starts among task t's 64 eigenvalues.  One wave per task, lane l
+// taking the task's eigenvalue l.
+__global__ __launch_bounds__(256) void k_grad_mask(int ntask, int nmat, const int* __restrict__ moff,
+                                                   const int* __restrict__ tprefix, int n_one,
+                                                   const double* __restrict__ w,
+                                                   const double* __restrict__ g,
+                                                   double* __restrict__ gm,
+                                                   int* __restrict__ tflag) {
+    const int t = uniform((int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    if (t >= ntask) return;
+    const int b = uniform(grad_task_matrix(tprefix, nmat, t));
+    const int off = moff ? uniform(moff[b]) : 0;
+    const int n = moff ? uniform(moff[b + 1]) - off : n_one;
+    const int t0 = tprefix ? uniform(tprefix[b]) : 0;
+    const int k = (t - t0) * kWave + lane;
     const double* W = w + off;
-    const double tol = cluster_tol(W, n);
-    const bool member = (k > 0 && W[k] - W[k - 1] <= tol) || (k + 1 < n && W[k + 1] - W[k] <= tol);
-    gm[j] = member ? 0.0 : g[j];
-    if ((k & (kWave - 1)) == 0) {
-        // the task's 64 eigenvalues, walked with a sliding window and no
-        // early exit, so the loads of the unrolled steps are in flight together
-        const int q1 = min(k + kWave, n);
-        int any = 0;
-        double prev = (k > 0) ? W[k - 1] : 0.0, cur = W[k];
-#pragma unroll 8
-        for (int q = k; q < q1; q++) {
-            const double next = (q + 1 < n) ? W[q + 1] : 0.0;
-            any |= (int)((q == 0 || cur - prev > tol) && (q + 1 < n && next - cur <= tol));
-            prev = cur;
-            cur = next;
-        }
-        tflag[t0 + k / kWave] = any;
+    bool starts = false;
+    if (k < n) {
+        const double tol = cluster_tol(W, n);
+        const double lo = (k > 0) ? W[k] - W[k - 1] : HUGE_VAL;
+        const double hi = (k + 1 < n) ? W[k + 1] - W[k] : HUGE_VAL;
+        gm[off + k] = (lo <= tol || hi <= tol) ? 0.0 : g[off + k];
+        starts = lo > tol && hi <= tol;
     }
+    const unsigned long long any = __ballot(starts);
+    if (lane == 0) tflag[t] = any ? 1 : 0;
 }
 
 // uniform in (-1, 1) from (row, member): the start vector of an iteration
@@ -929,40 +919,61 @@ __global__ __launch_bounds__(kWave) void k_grad_clusters(
     }
 }
 
-// dD_j, dE_j of every row: the sum over the matrix's tasks, in task order.
-__global__ void k_grad_reduce(int rows, int nmat, const int* __restrict__ moff,
-                              const int* __restrict__ tprefix, const long long* __restrict__ poff,
-                              int n_one, const double* __restrict__ partial,
-                              double* __restrict__ dd, double* __restrict__ de) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= rows) return;
-    int b = 0, off = 0, n = n_one, nt = (n_one + kWave - 1) / kWave;
-    long long po = 0;
-    if (moff) {
-        int lo = 0, hi = nmat;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (moff[mid] <= j) lo = mid;
-            else hi = mid;
-        }
-        b = lo;
-        off = moff[b];
-        n = moff[b + 1] - off;
-        nt = tprefix[b + 1] - tprefix[b];
-        po = poff[b];
-    }
-    const int k = j - off;
+// The per-matrix sums of the tasks' partial sums: block of 256 threads = 32
+// rows x 8 slices of the matrix's tasks; each slice sums its contiguous run
+// of tasks in order and the slices meet in LDS in slice order, so the result
+// does not depend on the launch shape.
+constexpr int kReduceRows = 32, kReduceSlices = 8;
+__global__ __launch_bounds__(kReduceRows * kReduceSlices) void k_grad_reduce(
+    int rows, int nmat, const int* __restrict__ moff, const int* __restrict__ tprefix,
+    const long long* __restrict__ poff, int n_one, const double* __restrict__ partial,
+    double* __restrict__ dd, double* __restrict__ de) {
+    __shared__ double red[2][kReduceSlices][kReduceRows];
+    const int r = threadIdx.x % kReduceRows, sl = threadIdx.x / kReduceRows;
+    const int j = blockIdx.x * kReduceRows + r;
     double sd = 0.0, se = 0.0;
-    // unrolled so that the loads of several tasks are in flight; the adds
-    // keep the task order
-#pragma unroll 8
-    for (int t = 0; t < nt; t++) {
-        const double* p = partial + po + (long long)t * 2 * n;
-        sd += p[k];
-        se += p[n + k];
+    int k = 0, n = 1;
+    if (j < rows) {
+        int b = 0, off = 0, nt = (n_one + kWave - 1) / kWave;
+        long long po = 0;
+        n = n_one;
+        if (moff) {
+            int lo = 0, hi = nmat;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (moff[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            b = lo;
+            off = moff[b];
+            n = moff[b + 1] - off;
+            nt = tprefix[b + 1] - tprefix[b];
+            po = poff[b];
+        }
+        k = j - off;
+        const int ta = (int)(((long long)nt * sl) / kReduceSlices);
+        const int tb = (int)(((long long)nt * (sl + 1)) / kReduceSlices);
+        // unrolled so that the loads of several tasks are in flight; the adds
+        // keep the task order
+#pragma unroll 4
+        for (int t = ta; t < tb; t++) {
+            const double* p = partial + po + (long long)t * 2 * n;
+            sd += p[k];
+            se += p[n + k];
+        }
     }
-    dd[j] = sd;
-    de[j] = (k < n - 1) ? se : 0.0;
+    red[0][sl][r] = sd;
+    red[1][sl][r] = se;
+    __syncthreads();
+    if (sl == 0 && j < rows) {
+        double ad = red[0][0][r], ae = red[1][0][r];
+        for (int q = 1; q < kReduceSlices; q++) {
+            ad += red[0][q][r];
+            ae += red[1][q][r];
+        }
+        dd[j] = ad;
+        de[j] = (k < n - 1) ? ae : 0.0;
+    }
 }
 
 }  // namespace brdc

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(1024) void k_norm(int n, const int* __restrict__ mo
         }
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            nonfinite |= !isfinite(x[u]) | !isfinite(y[u]);
+            nonfinite |= (int)!isfinite(x[u]) | (int)!isfinite(y[u]);
             m = fmax(m, fmax(fabs(x[u]), fabs(y[u])));
         }
     }
@@ -1026,11 +1026,25 @@ struct PoleView {
 
 constexpr int kBatch = 8;
 
+// Reciprocal of the pole terms of a secular evaluation (the poles outside
+// the root's exclusion window): v_rcp_f64 and one Newton step, relative error
+// below 2^-50 (a few ulp).  The stopping bound of each pass, eps * erretm
+// with erretm >= 8 A + sum_j |j - ii| |term_j| >= 9 A over these poles (A =
+// sum of |term_j|), stays above that rounding.
+__device__ __forceinline__ double rcp_stream(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Sums of one side of the pole list for this lane's strided poles
 // j = j_start, j_start + L, ... < j_end: S += z_j^2/delta_j, D += (z_j/delta_j)^2,
 // E += j z_j^2/delta_j.  Poles are sorted and every iterate stays inside its
 // bracket, so all poles below the origin give delta < 0 (psi side) and all
-// above give delta > 0 (phi side): the two sides are two plain loops.
+// above give delta > 0 (phi side): the two sides are two plain loops.  Four
+// poles share one reciprocal: with u = x0 x1 and v = x2 x3, 1/x0 = x1 v /
+// (u v) and so on (deflation keeps every delta above about 1e-16 of the
+// spectrum's width and below it, so u v neither overflows nor underflows),
+// and a batch's index-weighted sum is jd T + L sum_b b t_b.
 __device__ __forceinline__ void secular_side(const double2* __restrict__ dz, int j, int jend,
                                              int L, double dorg, double tau, double& S,
                                              double& D, double& E) {
@@ -1040,19 +1054,32 @@ __device__ __forceinline__ void secular_side(const double2* __restrict__ dz, int
         double2 p[kBatch];
 #pragma unroll
         for (int b = 0; b < kBatch; b++) p[b] = dz[j + b * L];
+        double T = 0.0, B = 0.0;
 #pragma unroll
-        for (int b = 0; b < kBatch; b++) {
-            const double r = rcp_nr((p[b].x - dorg) - tau);
-            const double term = p[b].y * r;
-            S += term;
-            D = fma(term, r, D);
-            E = fma(jd, term, E);
-            jd += dL;
+        for (int b = 0; b < kBatch; b += 4) {
+            double dl[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) dl[q] = (p[b + q].x - dorg) - tau;
+            const double u = dl[0] * dl[1], v = dl[2] * dl[3];
+            const double ruv = rcp_stream(u * v);
+            const double ru = v * ruv, rv = u * ruv;
+            const double rr[4] = {dl[1] * ru, dl[0] * ru, dl[3] * rv, dl[2] * rv};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double t = p[b + q].y * rr[q];
+                T = fma(p[b + q].y, rr[q], T);
+                D = fma(t, rr[q], D);
+                if (b + q > 0) B = fma((double)(b + q), t, B);
+            }
         }
+        S += T;
+        E = fma(jd, T, E);
+        E = fma(dL, B, E);
+        jd += kBatch * dL;
     }
     for (; j < jend; j += L) {
         const double2 p = dz[j];
-        const double r = rcp_nr((p.x - dorg) - tau);
+        const double r = rcp_stream((p.x - dorg) - tau);
         const double term = p.y * r;
         S += term;
         D = fma(term, r, D);
@@ -1414,17 +1441,6 @@ constexpr int kStageChunk = 128;  // poles staged per chunk (two per lane)
 struct StreamAcc {  // one side's sums over a pole range
     double S, D, J;
 };
-
-// Reciprocal of the streamed pole terms (the poles outside every root's
-// exclusion window): v_rcp_f64 and one Newton step, relative error below
-// 2^-50 (a few ulp).  The stopping bound of each pass, eps * erretm with
-// erretm >= 8 A + sum_j |j - ii| |term_j| >= 9 A over these poles (A = sum of
-// |term_j|), stays above that rounding, and with K >= kstream poles the
-// index-weighted part is many times A.
-__device__ __forceinline__ double rcp_stream(double x) {
-    const double r = __builtin_amdgcn_rcp(x);
-    return fma(r, fma(-x, r, 1.0), r);
-}
 
 // A load that names the global address space.  Through a generic pointer the
 // compiler emits a flat load, which also counts against the LDS wait counter:
