@@ -117,6 +117,36 @@ __device__ __forceinline__ double reduce_scatter32(double (&v)[32], int lane) {
     return t + dpp_f64<0xB1>(t);  // the pair i, i ^ 1
 }
 
+// Sum over the wave's 64 lanes of each of 16 values v[0..15], scattered:
+// returns in lane L the total of value L / 4 (the four lanes of a quad
+// agree).  The halvings pair lanes across bit 5, 4, 3, 2 as above, then the
+// quad adds its lanes i ^ 2 and i ^ 1: the same partners in the same order as
+// reduce_scatter32, so each total is the same sum.
+__device__ __forceinline__ double reduce_scatter16(double (&v)[16], int lane) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // bit 5
+        lane_swap_f64<true>(v[i], v[8 + i]);
+        v[i] = v[i] + v[8 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {  // bit 4
+        lane_swap_f64<false>(v[i], v[4 + i]);
+        v[i] = v[i] + v[4 + i];
+    }
+    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {  // bit 3, partner i ^ 15
+        const double keep = b3 ? v[2 + i] : v[i];
+        const double send = b3 ? v[i] : v[2 + i];
+        v[i] = keep + dpp_f64<0x140>(send);
+    }
+    const double keep = b2 ? v[1] : v[0];  // bit 2, partner i ^ 7
+    const double send = b2 ? v[0] : v[1];
+    double t = keep + dpp_f64<0x141>(send);
+    t = t + dpp_f64<0x4E>(t);     // partner i ^ 2
+    return t + dpp_f64<0xB1>(t);  // partner i ^ 1
+}
+
 // Work items of the eigenvector kernel (k_grad_vectors).  A task is 64
 // eigenvalues of one matrix, one per lane; its passes are
 //   F  the stationary pivot chain D+ (top down), checkpointed at every
@@ -163,7 +193,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     const double* __restrict__ g, double* __restrict__ cpbuf, int* __restrict__ rarr,
     double* __restrict__ ndarr, double* __restrict__ nuarr, double* __restrict__ garr,
     int* __restrict__ rgarr, double* __restrict__ partial, double scan_big) {
-    __shared__ double stage[kGradWaves][2][kWin];
+    __shared__ double stage[kGradWaves][4][kWin];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = uniform((int)(threadIdx.x / kWave));
     const int wave = uniform((int)(blockIdx.x * kGradWaves + wv));
@@ -171,6 +201,8 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
     const double safmin = 0x1p-1022;
     double* sD = stage[wv][0];
     double* sE = stage[wv][1];
+    double* sE2 = stage[wv][2];  // E^2
+    double* sH = stage[wv][3];   // E^2 / (-pivmin), rounded as the guard step forms it
     for (int round = 0;; round++) {
         const int q = round * nwave + ((round & 1) ? nwave - 1 - wave : wave);
         if (q >= nitem) break;  // positions grow with the round
@@ -242,7 +274,10 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         // clamped to the matrix, E zero outside rows 0..n-2 (rows outside the
         // matrix only feed values that are never used, except that the zero E
         // of row n-1 starts the top segment's D- chain).
+        const double rpv = __builtin_amdgcn_rcp(-pivmin);
+        const double wpv = fma(pivmin, rpv, 2.0);
         int wbase = -(1 << 30);
+        int so = 0;  // the loaded segment's offset in the window
         auto window = [&](int lo, bool up) {
             if (lo - 1 >= wbase && lo + kSeg + 1 <= wbase + kWin) return;
             wbase = up ? lo - 1 : lo + kSeg + 1 - kWin;
@@ -263,8 +298,11 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
+                const double b = ve[q] * ve[q];
                 sD[lane + q * kWave] = vd[q];
                 sE[lane + q * kWave] = ve[q];
+                sE2[lane + q * kWave] = b;
+                sH[lane + q * kWave] = b * rpv;
             }
             __builtin_amdgcn_wave_barrier();
         };
@@ -272,24 +310,24 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
         auto load_seg = [&](int lo, bool up) {
             window(lo, up);
             const int o = lo - wbase;
+            so = o;
 #pragma unroll
             for (int j = 0; j <= kSeg; j++) dv[j] = sD[o + j];
 #pragma unroll
             for (int j = 0; j < kSeg; j++) ev[j] = sE[o + j];
             eprev = sE[o - 1];
         };
-        // One step of a pivot chain: dk - mu - ek^2 / guard(x).  The chain
-        // carries unguarded values and the guard acts as a select of the step
-        // from -pivmin (rg below), decided from x while its reciprocal is in
-        // flight; 1/x is v_rcp_f64 refined by one Newton factor 2 - x r.
-        const double rpv = __builtin_amdgcn_rcp(-pivmin);
-        const double wpv = fma(pivmin, rpv, 2.0);
-        auto step = [&](double ek, double x, double dk) {
-            const double b = ek * ek, c = dk - mu;
+        // One step of a pivot chain: dk - mu - ek^2 / guard(x), row k = so + j
+        // of the window (E^2 and E^2 / (-pivmin) come from the window).  The
+        // chain carries unguarded values and the guard acts as a select of the
+        // step from -pivmin (rg below), decided from x while its reciprocal is
+        // in flight; 1/x is v_rcp_f64 refined by one Newton factor 2 - x r.
+        auto stepw = [&](int j, double x, double dk) {
+            const double b = sE2[so + j], h = sH[so + j], c = dk - mu;
             const double r = __builtin_amdgcn_rcp(x);
             const double w = fma(-x, r, 2.0);
             const double y = fma(-(b * r), w, c);
-            const double rg = fma(-(b * rpv), wpv, c);
+            const double rg = fma(-h, wpv, c);
             return (fabs(x) < pivmin) ? rg : y;
         };
         auto guard = [&](double x) { return pivot_guard(x, pivmin); };
@@ -299,8 +337,25 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
 #pragma unroll
             for (int j = 0; j < kSeg; j++) {
                 P[j] = guard(x);
-                x = step(ev[j], x, dv[j + 1]);
+                x = stepw(j, x, dv[j + 1]);
             }
+            return x;
+        };
+        // The same step with the guard applied to x first (the same values:
+        // the step from -pivmin is the guarded one), also giving 1 / guard(x):
+        // fewer operations, one more on the chain.
+        auto stepg = [&](int j, double x, double dk, double& rinv) {
+            const double c = dk - mu;
+            const double xg = guard(x);
+            const double r = __builtin_amdgcn_rcp(xg);
+            const double w = fma(-xg, r, 2.0);
+            rinv = r * w;
+            return fma(-(sE2[so + j] * r), w, c);
+        };
+        // seg_dp with P[j] = 1 / guard(D+ of row lo+j) instead
+        auto seg_dp_r = [&](double x) {
+#pragma unroll
+            for (int j = 0; j < kSeg; j++) x = stepg(j, x, dv[j + 1], P[j]);
             return x;
         };
         // M[j] = D- of row lo+j: the top segment starts at D_{n-1} - mu, the
@@ -311,9 +366,20 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
             double x = top ? 1.0 : xhi;
 #pragma unroll
             for (int j = kSeg - 1; j >= 0; j--) {
-                x = step(ev[j], x, dv[j]);
+                x = stepw(j, x, dv[j]);
                 M[j] = guard(x);
             }
+        };
+        // seg_dm with M[j] = 1 / guard(D- of row lo+j) instead: each step's
+        // reciprocal of its input is the previous row's, the last one apart
+        auto seg_dm_r = [&](int len, bool top, double xhi) {
+            double x = top ? 1.0 : xhi;
+            double dummy;
+#pragma unroll
+            for (int j = kSeg - 1; j >= 0; j--) x = stepg(j, x, dv[j], (j < kSeg - 1) ? M[j + 1] : dummy);
+            const double xg = guard(x);
+            const double r = __builtin_amdgcn_rcp(xg);
+            M[0] = r * fma(-xg, r, 2.0);
         };
         // gamma over segments [sa, sb), bottom up, rows high to low: the first
         // strictly smaller |gamma| wins (the sequential scan's tie rule).
@@ -333,14 +399,29 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 const double cpc = cpn;
                 if (s > sa) cpn = cpF[at(s - 1)];  // next segment's checkpoint in flight
                 load_seg(lo, false);
-                seg_dp(cpc);
                 seg_dm(len, s == S - 1, xhi);
+                // D+ top down with gamma on the fly (P[j] = 1 / guard(D+_j) for
+                // the transfers): the segment's best keeps its highest row among
+                // equal values (<=, rows rising), and a lower segment replaces
+                // the best only when strictly smaller, as the bottom-up scan
+                {
+                    double x = cpc, sg = HUGE_VAL;
+                    int srow = rbest;
 #pragma unroll
-                for (int j = kSeg - 1; j >= 0; j--) {
-                    const double gam = fabs(P[j] + M[j] - (dv[j] - mu));
-                    const bool better = (j < len) && gam < gbest;
-                    gbest = better ? gam : gbest;
-                    rbest = better ? lo + j : rbest;
+                    for (int j = 0; j < kSeg; j++) {
+                        const double xg = guard(x);
+                        const double gam = fabs(xg + M[j] - (dv[j] - mu));
+                        const bool better = (j < len) && gam <= sg;
+                        sg = better ? gam : sg;
+                        srow = better ? lo + j : srow;
+                        const double r = __builtin_amdgcn_rcp(xg);
+                        const double w = fma(-xg, r, 2.0);
+                        P[j] = r * w;
+                        x = fma(-(sE2[so + j] * r), w, dv[j + 1] - mu);
+                    }
+                    const bool better = sg < gbest;
+                    gbest = better ? sg : gbest;
+                    rbest = better ? srow : rbest;
                 }
                 if (store_b) cpB[at(s)] = M[0];
                 xhi = M[0];
@@ -349,7 +430,7 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                     bool okd = true;
 #pragma unroll
                     for (int j = kSeg - 1; j >= 0; j--) {
-                        pd *= -(ev[j] * rcp_chain(P[j]));
+                        pd *= -(ev[j] * P[j]);
                         qd = fma(pd, pd, qd);
                         okd = okd && fabs(pd) <= scan_big;
                     }
@@ -434,11 +515,11 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 const double cpc = cpn;
                 if (s > s_lo) cpn = cpF[at(s - 1)];
                 load_seg(lo, false);
-                seg_dp(cpc);
+                seg_dp_r(cpc);
 #pragma unroll
                 for (int j = kSeg - 1; j >= 0; j--) {
                     const int k = lo + j;
-                    const double zn = -(ev[j] * rcp_chain(P[j])) * z;
+                    const double zn = -(ev[j] * P[j]) * z;
                     z = (j >= len || k >= r) ? 1.0 : zn;
                     nrm += (j < len && k < r) ? zn * zn : 0.0;
                 }
@@ -453,12 +534,12 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 const double cbc = cbn;
                 if (s + 2 < S) cbn = cpB[at(s + 2)];
                 load_seg(lo, true);
-                seg_dm(len, s == S - 1, cbc);
+                seg_dm_r(len, s == S - 1, cbc);
 #pragma unroll
                 for (int j = 0; j < kSeg; j++) {
                     const int k = lo + j;
                     const double ek = (j == 0) ? eprev : ev[j - 1];
-                    const double zn = -(ek * rcp_chain(M[j])) * z;
+                    const double zn = -(ek * M[j]) * z;
                     const bool on = (j < len) && k > r;
                     z = (k <= r) ? 1.0 : (j < len ? zn : z);
                     nrm += on ? zn * zn : 0.0;
@@ -621,14 +702,14 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
                 z2 = ldZ(s + 3);
             }
             load_seg(lo, true);
-            seg_dp(cF);
-            seg_dm(len, s == S - 1, cB);
+            seg_dp_r(cF);
+            seg_dm_r(len, s == S - 1, cB);
             // below r: downward from z_{lo+16} (or from z_r = 1 inside the segment)
             double x = zhi;
 #pragma unroll
             for (int j = kSeg - 1; j >= 0; j--) {
                 const int k = lo + j;
-                x = (j >= len || k >= r) ? 1.0 : -(ev[j] * rcp_chain(P[j])) * x;
+                x = (j >= len || k >= r) ? 1.0 : -(ev[j] * P[j]) * x;
                 P[j] = x;
             }
             // at and above r: upward from z_lo (or from z_r = 1)
@@ -636,31 +717,35 @@ __global__ __launch_bounds__(kWave * kGradWaves) void k_grad_vectors(
 #pragma unroll
             for (int j = 0; j < kSeg; j++) {
                 const int k = lo + j;
-                if (j > 0) x = (k <= r) ? 1.0 : -(ev[j - 1] * rcp_chain(M[j])) * x;
+                if (j > 0) x = (k <= r) ? 1.0 : -(ev[j - 1] * M[j]) * x;
                 else x = (k <= r) ? 1.0 : zlo;
                 M[j] = (k < r) ? P[j] : x;  // z of row lo+j
             }
-            // v[j] = dD contribution of row lo+j, v[16+j] = half the dE
-            // contribution (the factor 2 is applied to the row total: scaling
-            // by 2 is exact, so the sums are those of the doubled terms)
-            double v[2 * kSeg];
+            // the row sums over the wave's eigenvalues, dD then dE, each a
+            // reduce-scatter of 16 values: afterwards lane L holds the total
+            // of row L / 4.  dE is summed as sc z_k z_{k+1} and doubled at the
+            // end (scaling by 2 is exact, so this is the sum of the doubled
+            // terms).
+            static_assert(kSeg == 16, "the reduce-scatter takes 16 values");
+            const int row = lane >> 2;
+            const bool writer = (lane & 3) == 0 && row < len;
+            {
+                double v[kSeg];
 #pragma unroll
-            for (int j = 0; j < kSeg; j++) {
-                const double zn = (j + 1 < kSeg) ? M[j + 1] : zhi;
-                const double znext = (j + 1 < len) ? zn : ((j + 1 == len) ? zhi : 0.0);
-                const double t = sc * M[j];
-                v[j] = t * M[j];
-                v[kSeg + j] = t * znext;
+                for (int j = 0; j < kSeg; j++) v[j] = (sc * M[j]) * M[j];
+                const double total = reduce_scatter16(v, lane);
+                if (writer) part[lo + row] = total;
             }
-            // reduce-scatter over the wave: afterwards lane L holds the total
-            // of value L / kShare
-            static_assert(2 * kSeg == 32, "the reduce-scatter takes 32 values");
-            constexpr int kShare = kWave / (2 * kSeg);  // lanes holding each total
-            const double total = reduce_scatter32(v, lane);
-            if ((lane & (kShare - 1)) == 0) {
-                const int idx = lane / kShare;
-                const int row = idx & (kSeg - 1);
-                if (row < len) part[(idx < kSeg ? 0 : n) + lo + row] = (idx < kSeg) ? total : 2.0 * total;
+            {
+                double v[kSeg];
+#pragma unroll
+                for (int j = 0; j < kSeg; j++) {
+                    const double zn = (j + 1 < kSeg) ? M[j + 1] : zhi;
+                    const double znext = (j + 1 < len) ? zn : ((j + 1 == len) ? zhi : 0.0);
+                    v[j] = (sc * M[j]) * znext;
+                }
+                const double total = reduce_scatter16(v, lane);
+                if (writer) part[n + lo + row] = 2.0 * total;
             }
         }
     }
