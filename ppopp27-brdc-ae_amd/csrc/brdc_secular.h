@@ -146,40 +146,35 @@ __host__ __device__ inline void secular_two(int i, const double* d, const double
 }
 
 // Three-pole step (the DLAED6 role): root of c + sum_k zz_k / (dd_k - t)
-// between dd[1], dd[2] (orgati) or dd[0], dd[1], starting from f(0) = finit.
+// between dd1, dd2 (orgati) or dd0, dd1, starting from f(0) = finit.  The
+// three poles are named scalars, not an array: a choice between two array
+// entries on orgati would become a computed index, which puts the array in
+// scratch memory.
 __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, double rho,
                                                      double dd0, double dd1, double dd2,
                                                      double zz0, double zz1, double zz2,
                                                      double finit, double* tau_out) {
-    const double dd[3] = {dd0, dd1, dd2};
-    const double zz[3] = {zz0, zz1, zz2};
     double lbd, ubd, tau = 0.0, a, b, c, eta, f, df, ddf, fc, erretm, temp;
-    double ds[3], zs[3];
     int info = 0;
     bool scale = false;
     double sclinv = 1.0;
-    if (orgati) {
-        lbd = dd[1];
-        ubd = dd[2];
-    } else {
-        lbd = dd[0];
-        ubd = dd[1];
-    }
+    lbd = orgati ? dd1 : dd0;
+    ubd = orgati ? dd2 : dd1;
     if (finit < 0.0)
         lbd = 0.0;
     else
         ubd = 0.0;
     if (kniter == 2) {
         if (orgati) {
-            temp = (dd[2] - dd[1]) / 2.0;
-            c = rho + fdiv(zz[0], (dd[0] - dd[1]) - temp);
-            a = c * (dd[1] + dd[2]) + zz[1] + zz[2];
-            b = c * dd[1] * dd[2] + zz[1] * dd[2] + zz[2] * dd[1];
+            temp = (dd2 - dd1) / 2.0;
+            c = rho + fdiv(zz0, (dd0 - dd1) - temp);
+            a = c * (dd1 + dd2) + zz1 + zz2;
+            b = c * dd1 * dd2 + zz1 * dd2 + zz2 * dd1;
         } else {
-            temp = (dd[0] - dd[1]) / 2.0;
-            c = rho + fdiv(zz[2], (dd[2] - dd[1]) - temp);
-            a = c * (dd[0] + dd[1]) + zz[0] + zz[1];
-            b = c * dd[0] * dd[1] + zz[0] * dd[1] + zz[1] * dd[0];
+            temp = (dd0 - dd1) / 2.0;
+            c = rho + fdiv(zz2, (dd2 - dd1) - temp);
+            a = c * (dd0 + dd1) + zz0 + zz1;
+            b = c * dd0 * dd1 + zz0 * dd1 + zz1 * dd0;
         }
         temp = rcp_nr(dev_max3(a, b, c));
         a *= temp;
@@ -193,11 +188,11 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
             tau = fdiv(2.0 * b, a + sqrt(fabs(a * a - 4.0 * b * c)));
         if (tau < lbd || tau > ubd)
             tau = (lbd + ubd) / 2.0;
-        if (dd[0] == tau || dd[1] == tau || dd[2] == tau) {
+        if (dd0 == tau || dd1 == tau || dd2 == tau) {
             tau = 0.0;
         } else {
-            temp = finit + fdiv(tau * zz[0], dd[0] * (dd[0] - tau)) +
-                   fdiv(tau * zz[1], dd[1] * (dd[1] - tau)) + fdiv(tau * zz[2], dd[2] * (dd[2] - tau));
+            temp = finit + fdiv(tau * zz0, dd0 * (dd0 - tau)) +
+                   fdiv(tau * zz1, dd1 * (dd1 - tau)) + fdiv(tau * zz2, dd2 * (dd2 - tau));
             if (temp <= 0.0)
                 lbd = tau;
             else
@@ -207,8 +202,8 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
         }
     }
     // rescale when a pole is within 2^-340 of tau, so 1/t^3 cannot overflow
-    temp = orgati ? fmin(fabs(dd[1] - tau), fabs(dd[2] - tau))
-                  : fmin(fabs(dd[0] - tau), fabs(dd[1] - tau));
+    temp = orgati ? fmin(fabs(dd1 - tau), fabs(dd2 - tau)) : fmin(fabs(dd0 - tau), fabs(dd1 - tau));
+    double ds0 = dd0, ds1 = dd1, ds2 = dd2, zs0 = zz0, zs1 = zz1, zs2 = zz2;
     if (temp <= 0x1p-340) {
         double sclfac;
         scale = true;
@@ -219,32 +214,29 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
             sclfac = 0x1p340;
             sclinv = 0x1p-340;
         }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            ds[k] = dd[k] * sclfac;
-            zs[k] = zz[k] * sclfac;
-        }
+        ds0 = dd0 * sclfac;
+        ds1 = dd1 * sclfac;
+        ds2 = dd2 * sclfac;
+        zs0 = zz0 * sclfac;
+        zs1 = zz1 * sclfac;
+        zs2 = zz2 * sclfac;
         tau *= sclfac;
         lbd *= sclfac;
         ubd *= sclfac;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            ds[k] = dd[k];
-            zs[k] = zz[k];
-        }
     }
     // the poles are fixed through the iteration: one reciprocal each
-    const Divisor rds[3] = {Divisor(ds[0]), Divisor(ds[1]), Divisor(ds[2])};
+    const Divisor rd0(ds0), rd1(ds1), rd2(ds2);
     fc = df = ddf = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const double r = rcp_nr(ds[k] - tau);
-        const double t1 = zs[k] * r, t2 = t1 * r, t3 = t2 * r;
-        fc += rds[k].div(t1);
+    auto term0 = [&](double ds, double zs, const Divisor& rd) {
+        const double r = rcp_nr(ds - tau);
+        const double t1 = zs * r, t2 = t1 * r, t3 = t2 * r;
+        fc += rd.div(t1);
         df += t2;
         ddf += t3;
-    }
+    };
+    term0(ds0, zs0, rd0);
+    term0(ds1, zs1, rd1);
+    term0(ds2, zs2, rd2);
     f = finit + tau * fc;
     if (fabs(f) <= 0.0)
         goto done;
@@ -253,14 +245,8 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
     else
         ubd = tau;
     for (int niter = 2; niter <= kLaed6MaxIt; niter++) {
-        double t1, t2;
-        if (orgati) {
-            t1 = ds[1] - tau;
-            t2 = ds[2] - tau;
-        } else {
-            t1 = ds[0] - tau;
-            t2 = ds[1] - tau;
-        }
+        const double t1 = (orgati ? ds1 : ds0) - tau;
+        const double t2 = (orgati ? ds2 : ds1) - tau;
         a = (t1 + t2) * f - t1 * t2 * df;
         b = t1 * t2 * f;
         c = f - (t1 + t2) * df + t1 * t2 * ddf;
@@ -275,22 +261,19 @@ __host__ __device__ __forceinline__ int secular_three(int kniter, bool orgati, d
         if (tau < lbd || tau > ubd)
             tau = (lbd + ubd) / 2.0;
         fc = erretm = df = ddf = 0.0;
-        bool hit = false;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            if (ds[k] - tau == 0.0) hit = true;
-        }
-        if (hit)
+        if (ds0 - tau == 0.0 || ds1 - tau == 0.0 || ds2 - tau == 0.0)
             goto done;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const double r = rcp_nr(ds[k] - tau);
-            const double u1 = zs[k] * r, u2 = u1 * r, u3 = u2 * r, u4 = rds[k].div(u1);
+        auto term = [&](double ds, double zs, const Divisor& rd) {
+            const double r = rcp_nr(ds - tau);
+            const double u1 = zs * r, u2 = u1 * r, u3 = u2 * r, u4 = rd.div(u1);
             fc += u4;
             erretm += fabs(u4);
             df += u2;
             ddf += u3;
-        }
+        };
+        term(ds0, zs0, rd0);
+        term(ds1, zs1, rd1);
+        term(ds2, zs2, rd2);
         f = finit + tau * fc;
         erretm = 8.0 * (fabs(finit) + fabs(tau) * erretm) + fabs(tau) * df;
         if (fabs(f) <= 4.0 * kEps * erretm || (ubd - lbd) <= 4.0 * kEps * fabs(tau))
@@ -563,6 +546,15 @@ struct SecularSolve {
             }
             ii = K - 1;
             dorg = di;
+            // the interior roots' fields, unused by the last root: every path
+            // sets the same fields (a field set on one path only can keep the
+            // solver state out of registers)
+            orgati = false;
+            zii2 = zi2;
+            dm1 = dip1;
+            zm12 = zip12;
+            dp1 = 0.0;
+            zp12 = 0.0;
             ev_org = dorg;
             ev_tau = tau;
             ev_exlo = K - 1;
