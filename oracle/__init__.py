@@ -69,13 +69,14 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(_ip)
 
 
-def query(n: int) -> tuple[int, int, int]:
-    """Workspace query (LWORK = -1): returns (info, lwmin, liwmin)."""
+def query(n: int, smlsiz: int = 0) -> tuple[int, int, int]:
+    """Workspace query (LWORK = -1) for leaves of at most smlsiz rows (0: the
+    default): returns (info, lwmin, liwmin)."""
     w = np.zeros(1)
     iw = np.zeros(1, dtype=np.int32)
     dd = np.zeros(max(n, 1))
     ee = np.zeros(max(n, 1))
-    info = lib().brdc_oracle_dstebr_ex(n, _p(dd), _p(ee), _p(w), -1, _p(iw), -1, 0, 0,
+    info = lib().brdc_oracle_dstebr_ex(n, _p(dd), _p(ee), _p(w), -1, _p(iw), -1, smlsiz, 0,
                                        None, 0, None)
     return info, int(w[0]), int(iw[0])
 
@@ -91,7 +92,7 @@ def dstebr(d, e, smlsiz: int = 0, nthreads: int = 0, stats: bool = False):
     e = np.array(e, dtype=np.float64, copy=True).reshape(-1)
     e_buf = np.zeros(max(n, 1))
     e_buf[: e.shape[0]] = e
-    info, lw, liw = query(n)
+    info, lw, liw = query(n, smlsiz)
     work = np.zeros(max(lw, 1))
     iwork = np.zeros(max(liw, 1), dtype=np.int32)
     if stats:
