@@ -98,6 +98,24 @@ def test_n65536_normal_vs_oracle_and_workspace(gpu, orc):
     assert brdc.workspace_bytes(n) < 1 << 30
 
 
+def test_n262144_invariants_and_closed_form(gpu):
+    # four times the largest order the reference measures: the Toeplitz matrix
+    # (root merge of 131072 poles) against its closed form, a normal matrix
+    # through sortedness and the trace and Frobenius invariants
+    n = 262144
+    d, e = generate("toeplitz", n, 7)
+    w, info, _ = solve(gpu, d, e)
+    k = np.arange(1, n + 1)
+    cf = np.sort(2.0 + 2.0 * np.cos(k * np.pi / (n + 1)))
+    assert info == 0 and rel_err(w, cf) <= 1e-14
+    d, e = generate("normal", n, 7)
+    w, info, _ = solve(gpu, d, e)
+    assert info == 0 and np.all(np.diff(w) >= 0.0)
+    scale = np.max(np.abs(w))
+    assert abs(w.sum() - d.sum()) <= 1e-13 * n * scale
+    assert abs((w ** 2).sum() - (d ** 2).sum() - 2.0 * (e ** 2).sum()) <= 1e-13 * (w ** 2).sum()
+
+
 def test_closed_form_families(gpu):
     import scipy.special
     n = 2049
