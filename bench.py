@@ -75,7 +75,7 @@ def profiled_traffic(kernel_class):
     return None, None
 
 
-FP64_CLASSES = ("secular", "secular_mid", "secular_stream", "wprod", "rows")
+FP64_CLASSES = ("secular", "secular_mid", "secular_stream", "wprod", "rows", "subtree")
 
 
 def class_rooflines(prof):

@@ -32,9 +32,10 @@ typedef enum brdc_status {
 
 /* Tuning knobs of the device path; zero-initialise for the defaults. */
 typedef struct brdc_options {
-    int leaf_max;           /* largest leaf (rows) of the tree, solved by the leaf
-                               kernel (implicit QL, one lane per leaf), 2..64;
-                               0 selects the default (16) */
+    int leaf_max;           /* largest leaf (rows) of the tree, solved by implicit
+                               QL, one lane per leaf, 2..64; 0 selects the
+                               default: 8 when the subtree kernel (below) takes
+                               the bottom of the tree, else 16 */
     int stream_min_k;       /* merges with at least this many non-deflated poles
                                go to the streamed secular kernel, 256..2^28;
                                0 selects the default (2048) */
@@ -44,11 +45,17 @@ typedef struct brdc_options {
     int wide_max_waves;     /* widest group (waves) of the product-form and
                                boundary-row kernels, 1, 2, 4 or 8; 0 selects
                                the default (8) */
-    int reserved[4];        /* must be zero */
+    int subtree_max_log2;   /* the bottom of the tree (nodes of at most 64 rows
+                               with all their leaves and merges) runs in one
+                               launch, a block per node, when there are at most
+                               2^subtree_max_log2 such nodes (a batch: over all
+                               its matrices), 1..30; -1 never; 0 selects the
+                               default (10) */
+    int reserved[3];        /* must be zero */
 } brdc_options;
-/* The three launch-shape knobs after leaf_max change which kernel solves a
- * merge and how sums are split over lanes, so results move within rounding
- * (not bit for bit) when they change; brdc.autotune picks them by timing. */
+/* The launch-shape knobs after leaf_max change which kernel solves a merge
+ * and how sums are split over lanes, so results move within rounding (not bit
+ * for bit) when they change; brdc.autotune picks them by timing. */
 
 /*
  * Host-array entry.  Replaces DSTEDC_DC_BR (cpu/src/dstedc_dc_br.f:7-8), the
@@ -155,19 +162,22 @@ int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void 
  * norm, prep_e, leaf, prepare, deflate, secular (merges below 2048 poles),
  * wprod, rows, finalize, secular_stream (merges of 2048 poles or more, every
  * evaluation after the first), secular_mid (their first, midpoint
- * evaluation) (BRDC_KERNEL_CLASSES of them), the summed milliseconds and
- * launch counts.
+ * evaluation), subtree (the leaves and every merge of nodes of at most 64
+ * rows, in one launch) (BRDC_KERNEL_CLASSES of them), the summed
+ * milliseconds and launch counts.
  */
-#define BRDC_KERNEL_CLASSES 11
+#define BRDC_KERNEL_CLASSES 12
 int brdc_profile_begin(void);
 int brdc_profile_end(double *ms_out, int *launches_out, int nclasses);
 
 /* As brdc_profile_end, split by merge level (the per-level values_*_ms of
  * TridiagMergeLevelDiagnostics): ms_out and launches_out hold (nlevels + 1)
  * rows of nclasses entries; row 0 gathers the launches before the merges
- * (norm, prep_e, leaf), row 1 + l the merges of level l (tree depth, 0 = the
- * root merge; for a batch, forest height l, 1 = one level above the leaves).
- * Levels beyond nlevels are dropped. */
+ * (norm, prep_e, leaf, subtree: the subtree launch holds the levels of nodes
+ * of at most 64 rows, which then have no row of their own), row 1 + l the
+ * merges of level l (tree depth, 0 = the root merge; for a batch, forest
+ * height l, 1 = one level above the leaves).  Levels beyond nlevels are
+ * dropped. */
 int brdc_profile_end_levels(double *ms_out, int *launches_out, int nclasses, int nlevels);
 
 /*

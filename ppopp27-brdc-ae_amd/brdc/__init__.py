@@ -56,10 +56,12 @@ def dstebr(d, e):
     return d, int(info)
 
 
-def workspace_bytes(n: int, leaf_max: int = 0) -> int:
+def workspace_bytes(n: int, leaf_max: int = 0, **tuning) -> int:
+    """Device workspace bytes of an order-n solve with these options (the
+    leaf bound, and the subtree knob that picks the default one)."""
     L = _lib.load()
     out = ctypes.c_size_t(0)
-    o = options(leaf_max)
+    o = options(leaf_max, **tuning)
     _lib.check(L.brdc_dstebr_device_workspace_ex(int(n), ctypes.byref(o), ctypes.byref(out)),
                "workspace query")
     return int(out.value)
@@ -76,7 +78,7 @@ class Solver:
         import torch
         self.n = int(n)
         self.opts = options(leaf_max, **tuning)
-        self.bytes = workspace_bytes(self.n, leaf_max)
+        self.bytes = workspace_bytes(self.n, leaf_max, **tuning)
         self.ws = torch.empty(max(self.bytes, 256), dtype=torch.uint8, device=device)
         self.info = torch.zeros(1, dtype=torch.int32, device=device)
         if self.ws.data_ptr() % 256:
@@ -232,7 +234,78 @@ def merge_stats(solver: Solver):
 #: read 3 + write 3 doubles) or once per solve (leaf: read d, e + write d,
 #: blo, bhi; norm: read d, e; prep_e: read e + write the scaled e)
 _BYTES_PER_ROW = {"prepare": 56, "deflate": 112, "finalize": 48, "leaf": 40, "norm": 16,
-                  "prep_e": 16}
+                  "prep_e": 16, "subtree": 40}
+# classes whose bytes are paid once per row (the others once per row of
+# every launched merge)
+_ROW_BYTE_CLASSES = ("leaf", "norm", "prep_e", "subtree")
+
+# leaf bounds when brdc_options.leaf_max is 0 (kDefaultLeaf, kSubtreeLeaf)
+# and the default of subtree_max_log2 (kSubMaxLog2), as in the HIP driver
+_DEFAULT_LEAF = 16
+_SUBTREE_LEAF = 8
+_SUB_MAX_LOG2 = 10
+
+
+def _tree_levels(n: int, leaf: int) -> int:
+    k = 0
+    while -(-n // (1 << k)) > leaf:
+        k += 1
+    return k
+
+
+def _subtree_depth(n: int, leaf: int) -> int:
+    """Depth of the subtree roots of an order-n tree (-1: none); mirrors
+    subtree_depth in the HIP driver."""
+    if n <= 1:
+        return -1
+    levels = _tree_levels(n, leaf)
+    for depth in range(levels):
+        if -(-n // (1 << depth)) <= _lib.SUBTREE_ROWS:
+            return depth if levels - depth <= 6 else -1
+    return -1
+
+
+def _subtree_limit(opts) -> int:
+    lg = opts.subtree_max_log2 or _SUB_MAX_LOG2
+    return 0 if lg < 0 else 1 << lg
+
+
+def effective_leaf(sizes, opts) -> int:
+    """The leaf bound a solve over matrices of these orders uses: the option
+    if set, else 8 when the subtree kernel takes the bottom of every tree at
+    that bound (at most 2^subtree_max_log2 subtree roots in all), else 16."""
+    if opts.leaf_max:
+        return int(opts.leaf_max)
+    roots = sum(1 << d for d in (_subtree_depth(int(n), _SUBTREE_LEAF) for n in sizes) if d >= 0)
+    return _SUBTREE_LEAF if 0 < roots <= _subtree_limit(opts) else _DEFAULT_LEAF
+
+
+def in_subtree(solver, stats) -> np.ndarray:
+    """Mask of the merge records (merge_stats) solved inside the subtree
+    kernel rather than by the level launches: the subtrees of every matrix
+    when their roots number at most 2^subtree_max_log2 (over a whole batch),
+    none otherwise."""
+    sizes = solver.sizes if isinstance(solver, BatchSolver) else np.array([solver.n])
+    leaf = effective_leaf(sizes, solver.opts)
+    depth = np.array([_subtree_depth(int(n), leaf) for n in sizes], dtype=np.int64)
+    lev = np.array([_tree_levels(int(n), leaf) if n > 0 else 0 for n in sizes], dtype=np.int64)
+    roots = int(np.where(depth >= 0, 2 ** np.maximum(depth, 0), 0).sum())
+    hs = np.where(depth >= 0, lev - depth, 0)
+    if roots > _subtree_limit(solver.opts):
+        hs[:] = 0
+    if isinstance(solver, BatchSolver):
+        mat = np.searchsorted(solver.offsets, stats["start"], side="right") - 1
+        return stats["level"] <= hs[mat]
+    return (hs[0] > 0) & (stats["level"] >= lev[0] - hs[0])
+
+
+def _root_merges(solver, stats) -> np.ndarray:
+    """Mask of the records that are a matrix's root merge (no boundary rows)."""
+    if isinstance(solver, BatchSolver):
+        return np.isin(stats["start"], solver.offsets[:-1]) & \
+            (stats["size"] == solver.sizes[np.searchsorted(solver.offsets, stats["start"],
+                                                           side="right") - 1])
+    return stats["level"] == 0
 
 
 def profile_kernels(solver: Solver, problems, stream=None):
@@ -257,7 +330,7 @@ def profile_kernels(solver: Solver, problems, stream=None):
     tot_ms = np.zeros(nc)
     tot_n = np.zeros(nc, dtype=np.int64)
     flops = {"secular": 0.0, "secular_stream": 0.0, "secular_mid": 0.0, "wprod": 0.0,
-             "rows": 0.0}
+             "rows": 0.0, "subtree": 0.0}
     nbytes = {k: 0.0 for k in _BYTES_PER_ROW}
     st = stream if stream is not None else torch.cuda.current_stream()
     for (w, d, e) in problems:
@@ -272,22 +345,28 @@ def profile_kernels(solver: Solver, problems, stream=None):
         tot_ms += ms
         tot_n += cnt
         stats = solver.merge_stats(st)
+        sub = in_subtree(solver, stats)
         k = stats["k"].astype(np.float64)
         ev = stats["evals"].astype(np.float64)
         kstream = solver.opts.stream_min_k or _lib.STREAM_MIN_K
-        big = stats["k"] >= kstream
-        flops["secular"] += float(np.sum(7.0 * k[~big] * ev[~big]))
+        big = (stats["k"] >= kstream) & ~sub
+        small = (stats["k"] < kstream) & ~sub
+        flops["secular"] += float(np.sum(7.0 * k[small] * ev[small]))
         # every root of a streamed merge has its first evaluation in secular_mid
         flops["secular_stream"] += float(np.sum(7.0 * k[big] * (ev[big] - k[big])))
         flops["secular_mid"] += float(np.sum(4.0 * k[big] * k[big]))
-        nonroot = (stats["level"] > 0) & (stats["k"] > 2)
-        flops["wprod"] += float(np.sum(3.0 * k[nonroot] ** 2))
-        flops["rows"] += float(np.sum(9.0 * k[nonroot] ** 2))
-        # every merge moves its rows once: the rows of all merges together
-        merged_rows = float(stats["size"].sum())
+        root = _root_merges(solver, stats)
+        nonroot = ~root & (stats["k"] > 2)
+        flops["wprod"] += float(np.sum(3.0 * k[nonroot & ~sub] ** 2))
+        flops["rows"] += float(np.sum(9.0 * k[nonroot & ~sub] ** 2))
+        flops["subtree"] += float(np.sum(7.0 * k[sub] * ev[sub]) +
+                                  np.sum(12.0 * k[nonroot & sub] ** 2))
+        # every launched merge moves its rows once; the subtree kernel reads
+        # and writes each of its rows once
+        merged_rows = float(stats["size"][~sub].sum())
         rows = float(getattr(solver, "rows", getattr(solver, "n", 0)))
         for kname, b in _BYTES_PER_ROW.items():
-            nbytes[kname] += b * (rows if kname in ("leaf", "norm", "prep_e") else merged_rows)
+            nbytes[kname] += b * (rows if kname in _ROW_BYTE_CLASSES else merged_rows)
     kern = {}
     for i, name in enumerate(_lib.KERNEL_CLASSES):
         if tot_n[i]:

@@ -27,7 +27,7 @@ EXPORTS = ("brdc_dstebr", "brdc_dstebr_", "dstedc_dc_br_", "brdc_dstebr_device_w
            "brdc_batch_merge_stats")
 
 KERNEL_CLASSES = ("norm", "prep_e", "leaf", "prepare", "deflate", "secular", "wprod", "rows",
-                  "finalize", "secular_stream", "secular_mid")
+                  "finalize", "secular_stream", "secular_mid", "subtree")
 # ints per merge record of brdc_merge_stats / brdc_batch_merge_stats
 MERGE_STAT_FIELDS = 9
 MERGE_STAT_NAMES = ("level", "start", "size", "k", "givens", "evals", "tiny", "evals_max",
@@ -38,6 +38,8 @@ NOALLOC_BYTES = 20 << 20
 # merges of at least this many poles go to the streamed secular kernel
 # (kStreamMinK in csrc/brdc.hip)
 STREAM_MIN_K = 2048
+# largest node (rows) solved inside the subtree kernel, with all of its leaves
+SUBTREE_ROWS = 64
 
 
 class Options(ctypes.Structure):
@@ -45,7 +47,7 @@ class Options(ctypes.Structure):
     plus reserved words."""
     _fields_ = [("leaf_max", ctypes.c_int), ("stream_min_k", ctypes.c_int),
                 ("secular_lanes_log2", ctypes.c_int), ("wide_max_waves", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("subtree_max_log2", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -128,12 +130,13 @@ TUNING_FIELDS = ("stream_min_k", "secular_lanes_log2", "wide_max_waves")
 
 
 def options(leaf_max: int = 0, stream_min_k: int = 0, secular_lanes_log2: int = 0,
-            wide_max_waves: int = 0) -> Options:
+            wide_max_waves: int = 0, subtree_max_log2: int = 0) -> Options:
     o = Options()
     o.leaf_max = int(leaf_max)
     o.stream_min_k = int(stream_min_k)
     o.secular_lanes_log2 = int(secular_lanes_log2)
     o.wide_max_waves = int(wide_max_waves)
+    o.subtree_max_log2 = int(subtree_max_log2)
     return o
 
 

@@ -10,7 +10,11 @@ tile/launch knobs of brdc_options.
 * secular_lanes_log2: the secular lanes a level aims to keep in flight, from
   which the lanes per root follow (more lanes: shorter passes, more waves);
 * wide_max_waves: the widest group of the product-form and boundary-row
-  kernels (waves splitting one 64-item group's stream).
+  kernels (waves splitting one 64-item group's stream);
+* subtree_max_log2: up to how many subtree roots (nodes of at most 64 rows)
+  the bottom of the trees runs in the one-launch subtree kernel (latency
+  bound: it wins while the subtrees are few, the level launches win when
+  thousands of subtrees fill the device anyway); -1 never, 30 always.
 
 The best values depend on the size mix and the spectrum, so they are
 measured: each candidate is timed as a captured-graph replay of the actual
@@ -37,6 +41,7 @@ KNOB_CANDIDATES = {
     "stream_min_k": (2048, 1024, 4096),
     "secular_lanes_log2": (18, 17, 19),
     "wide_max_waves": (8, 4, 2),
+    "subtree_max_log2": (0, -1, 30),
 }
 
 
@@ -128,7 +133,7 @@ def tune_batched(d_cat, e_cat, sizes, candidates=CANDIDATES, reps: int = 5,
     return dict(res, cached=False)
 
 
-def tune_single(problems, candidates=(12, 16, 24, 32), reps: int = 5,
+def tune_single(problems, candidates=(8, 12, 16, 24, 32), reps: int = 5,
                 cache_path: str | None = None, knobs: bool = True) -> dict:
     """Pick the launch shape for single solves of one order n: problems is a
     list of (d, e) float64 device tensors of equal length (for instance one

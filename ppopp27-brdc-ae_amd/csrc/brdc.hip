@@ -23,11 +23,16 @@
 #include "brdc_deflate_wide.h"
 #include "brdc_grad.h"
 #include "brdc_kernels.h"
+#include "brdc_subtree.h"
 
 namespace brdc {
 namespace {
 
+// Leaf bound when brdc_options.leaf_max is 0: small leaves when the subtree
+// kernel solves the bottom of the tree (its extra merge levels cost little
+// there, and a leaf's QL chain is quadratic in its rows), else 16.
 constexpr int kDefaultLeaf = 16;
+constexpr int kSubtreeLeaf = 8;
 // merges up to this many poles are staged in LDS (4 waves x 640 x 16 B =
 // 40 KB per block, so the LDS admits the 3 blocks per CU that the secular
 // kernel's register budget allows)
@@ -45,10 +50,12 @@ constexpr int kStreamMinK = 2048;   // merges with at least this many poles are 
 // Launch-shape knobs (brdc_options; 0 there selects these defaults): the
 // streaming threshold, the secular lanes in flight, and the widest group of
 // the product-form and boundary-row kernels.
+constexpr int kSubMaxLog2 = 10;      // subtree launches of at most 2^10 blocks by default
 struct Tuning {
     int stream_min_k = kStreamMinK;
     int target_lanes = kTargetLanes;
     int wide_waves = kMaxWaves;
+    long long sub_max = 1LL << kSubMaxLog2;  // most subtree blocks (0: never)
 };
 constexpr int kMidMaxGroups = 1024; // midpoint-pass grid cap (tiles of 64 roots)
 
@@ -90,12 +97,12 @@ struct Layout {
     size_t P, rend, fl;
     size_t wrec, wtol;  // chunk records and tolerances of the wide deflation
     size_t counts, info;
-    size_t plan, moff, tpre, poff, cpof, gitm;  // batch only
+    size_t plan, moff, tpre, poff, cpof, gitm, srec;  // batch only
     size_t total;
 };
 
 Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items,
-                   size_t grad_items = 0) {
+                   size_t grad_items = 0, size_t subtrees = 0) {
     Layout L{};
     const size_t nd = (size_t)std::max(rows, 1) * sizeof(double);
     const size_t ni = (size_t)std::max(rows, 1) * sizeof(int);
@@ -149,8 +156,20 @@ Layout make_layout(int rows, int nmat, size_t merge_records, size_t plan_items,
     L.poff = plan_items ? take(((size_t)nmat + 1) * sizeof(long long)) : 0;
     L.cpof = plan_items ? take(((size_t)nmat + 1) * sizeof(long long)) : 0;
     L.gitm = plan_items ? take(std::max<size_t>(grad_items, 1) * sizeof(int4)) : 0;
+    L.srec = plan_items ? take(std::max<size_t>(subtrees, 1) * kSubMaxLevels * sizeof(int)) : 0;
     L.total = off;
     return L;
+}
+
+// Depth of the subtree roots of a matrix of order n with leaves at depth
+// `levels`: the shallowest depth whose nodes have at most kSubRows rows, or
+// -1 when no merge fits (a subtree holds at least one merge and at most
+// kSubMaxLevels merge levels, which leaves of two or more rows guarantee).
+inline int subtree_depth(int n, int levels) {
+    for (int depth = 0; depth < levels; depth++)
+        if (max_node_size(n, depth) <= kSubRows)
+            return (levels - depth <= kSubMaxLevels) ? depth : -1;
+    return -1;
 }
 
 // One matrix of order n: merge records at index (2^depth - 1) + i.
@@ -159,19 +178,34 @@ Layout tree_layout(int n, int leaf) {
     return make_layout(n, 1, (size_t)1 << levels, 0);
 }
 
-int leaf_of(const brdc_options* o) {
-    if (!o || o->leaf_max == 0) return kDefaultLeaf;
-    return o->leaf_max;
-}
-
 Tuning tuning_of(const brdc_options* o) {
     Tuning t;
     if (!o) return t;
     if (o->stream_min_k) t.stream_min_k = o->stream_min_k;
     if (o->secular_lanes_log2) t.target_lanes = 1 << o->secular_lanes_log2;
     if (o->wide_max_waves) t.wide_waves = o->wide_max_waves;
+    if (o->subtree_max_log2) t.sub_max = (o->subtree_max_log2 < 0) ? 0 : 1LL << o->subtree_max_log2;
     return t;
 }
+
+// Subtree roots of an order-n tree with leaves of at most `leaf` rows (0
+// when no merge fits a subtree).
+long long subtree_roots(int n, int leaf) {
+    if (n <= 1) return 0;
+    const int depth = subtree_depth(n, tree_levels(n, leaf));
+    return depth >= 0 ? 1LL << depth : 0;
+}
+
+// The leaf bound of a solve over matrices of the given orders (one matrix,
+// or a batch): the option if set, else kSubtreeLeaf when the subtree kernel
+// takes the bottom of every tree at that bound, else kDefaultLeaf.
+int leaf_of(const brdc_options* o, const int* sizes, int nmat) {
+    if (o && o->leaf_max != 0) return o->leaf_max;
+    long long roots = 0;
+    for (int b = 0; b < nmat; b++) roots += subtree_roots(sizes[b], kSubtreeLeaf);
+    return (roots > 0 && roots <= tuning_of(o).sub_max) ? kSubtreeLeaf : kDefaultLeaf;
+}
+int leaf_of(const brdc_options* o, int n) { return leaf_of(o, &n, 1); }
 
 bool options_ok(const brdc_options* o) {
     if (!o) return true;
@@ -182,6 +216,7 @@ bool options_ok(const brdc_options* o) {
         return false;
     const int w = o->wide_max_waves;
     if (w != 0 && w != 1 && w != 2 && w != 4 && w != 8) return false;
+    if (o->subtree_max_log2 < -1 || o->subtree_max_log2 > 30) return false;
     for (int r : o->reserved)
         if (r != 0) return false;
     return true;
@@ -201,7 +236,7 @@ T* at(void* ws, size_t off) {
 // MergeLevelTimer, gpu/src/tridiag_solver.cu:158-359): when enabled on the
 // calling host thread, every launch is bracketed by two events on its stream.
 enum KernelClass { kNorm, kPrepE, kLeaf, kPrepare, kDeflate, kSecular, kWprod, kRows, kFinalize,
-                   kSecularStream, kSecularMid, kNumClasses };
+                   kSecularStream, kSecularMid, kSubtree, kNumClasses };
 // Each mark also carries the merge level it ran for (tree depth, 0 = root
 // merge, or forest height for a batch; -1 before the merges).
 struct ProfMark {
@@ -277,6 +312,10 @@ struct Schedule {
     const int* moff = nullptr;  // device: first row of each matrix (batch), else nullptr
     Geo leaves{};
     int nleaves = 0, leaf_rows = 0;
+    // the bottom of the tree in k_subtree (nsub > 0): the leaves and every
+    // merge below the subtree roots, which `levels` then leaves out
+    SubGeo sub{};
+    int nsub = 0;
     std::vector<LevelRun> levels;
 };
 
@@ -288,7 +327,15 @@ Schedule tree_schedule(int n, int leaf, const Tuning& tune) {
     S.leaves = Geo{n, levels, nullptr};
     S.nleaves = 1 << levels;
     S.leaf_rows = max_node_size(n, levels);
-    for (int depth = levels - 1; depth >= 0; depth--) {
+    const int sub = subtree_depth(n, levels);
+    int top = levels;  // merges at depths below this run level by level
+    if (sub >= 0 && (1LL << sub) <= tune.sub_max) {
+        S.sub = SubGeo{Geo{n, sub, nullptr}, levels, nullptr};
+        S.nsub = 1 << sub;
+        S.nleaves = 0;
+        top = sub;
+    }
+    for (int depth = top - 1; depth >= 0; depth--) {
         LevelRun r;
         r.geo = Geo{n, depth, nullptr};
         r.merges = 1 << depth;
@@ -332,7 +379,15 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
     double* sd[2] = {at<double>(ws, L.sd[0]), at<double>(ws, L.sd[1])};
     double* slo[2] = {at<double>(ws, L.slo[0]), at<double>(ws, L.slo[1])};
     double* shi[2] = {at<double>(ws, L.shi[0]), at<double>(ws, L.shi[1])};
-    {
+    if (S.nsub > 0) {
+        ProfScope ps(kSubtree, st);
+        hipLaunchKernelGGL(k_subtree, dim3(S.nsub), dim3(kSubThreads), 0, st, S.sub, S.moff,
+                           (const double*)d, (const double*)ew, (const double*)scal, d, sd[cur],
+                           slo[cur], shi[cur], counts_all, info,
+                           S.levels.empty() ? info_user : nullptr);
+        BRDC_LAUNCH_CHECK();
+    }
+    if (S.nleaves > 0) {
         ProfScope ps(kLeaf, st);
         // one lane per leaf, LDS sized by the largest leaf
         const int maxm = std::max(S.leaf_rows, 1);
@@ -510,9 +565,10 @@ int enqueue_solve(int n, double* d, const double* e, void* ws, const Layout& L, 
 // every matrix's merges h levels above its leaves, so one launch sequence
 // per height serves all matrices.  Built once on the host per size list.
 struct HostLevel {
-    int merges = 0, maxsz = 0;
+    int merges = 0;             // all merges of the height (one record each)
+    int launch = 0, maxsz = 0;  // the first `launch` of them run as level launches
     size_t plan_at = 0, counts_rec = 0;
-    bool any_nonroot = false;
+    bool any_nonroot = false;   // among the launched merges
 };
 
 }  // namespace
@@ -528,8 +584,12 @@ struct brdc_batch_plan {
     std::vector<int4> gitems;        // gradient items of the four launches, back to back
     int gcount[4] = {0, 0, 0, 0};    // items per launch
     int rows_max = 0;
-    std::vector<int4> items;         // leaves, then every level's merges
-    int nleaves = 0, leaf_rows = 0;
+    std::vector<int4> items;         // leaves, every level's merges, subtree roots
+    int nleaves = 0, leaf_rows = 0;  // leaves of the forest; the largest
+    int leaf_items = 0;              // leaves solved by the leaf kernel (first items)
+    size_t sub_at = 0;               // first subtree root in items
+    int nsub = 0;
+    std::vector<int> srec;           // per subtree root kSubMaxLevels record bases
     std::vector<brdc::HostLevel> levels;  // height 1 upward
     size_t merge_records = 0;
     brdc::Layout layout;
@@ -663,16 +723,22 @@ Schedule batch_schedule(const brdc_batch_plan& p, void* ws) {
     S.moff = at<int>(ws, p.layout.moff);
     const int4* plan = at<int4>(ws, p.layout.plan);
     S.leaves = Geo{p.rows, -1, plan};
-    S.nleaves = p.nleaves;
+    S.nleaves = p.leaf_items;
     S.leaf_rows = p.leaf_rows;
-    for (const HostLevel& h : p.levels) {
+    if (p.nsub > 0) {
+        S.sub = SubGeo{Geo{p.rows, -1, plan + p.sub_at}, 0, at<int>(ws, p.layout.srec)};
+        S.nsub = p.nsub;
+    }
+    for (size_t h = 0; h < p.levels.size(); h++) {
+        const HostLevel& hl = p.levels[h];
+        if (hl.launch == 0) continue;  // every merge of the height is inside a subtree
         LevelRun r;
-        r.geo = Geo{p.rows, -1, plan + h.plan_at};
-        r.merges = h.merges;
-        r.maxsz = h.maxsz;
-        r.counts_rec = h.counts_rec;
-        r.any_nonroot = h.any_nonroot;
-        r.report = (int)S.levels.size() + 1;
+        r.geo = Geo{p.rows, -1, plan + hl.plan_at};
+        r.merges = hl.launch;
+        r.maxsz = hl.maxsz;
+        r.counts_rec = hl.counts_rec;
+        r.any_nonroot = hl.any_nonroot;
+        r.report = (int)h + 1;
         S.levels.push_back(r);
     }
     return S;
@@ -701,7 +767,7 @@ const char* brdc_status_string(int status) {
 int brdc_dstebr_device_workspace_ex(int n, const brdc_options* opts, size_t* bytes) {
     if (!bytes) return BRDC_STATUS_INVALID_POINTER;
     if (n < 0 || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
-    *bytes = tree_layout(n, leaf_of(opts)).total;
+    *bytes = tree_layout(n, leaf_of(opts, n)).total;
     return BRDC_STATUS_SUCCESS;
 }
 
@@ -715,7 +781,7 @@ int brdc_dstebr_device_ex(int n, double* d_dev, const double* e_dev, void* works
     if (n < 0 || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
     if (n == 0) return BRDC_STATUS_SUCCESS;
     if (!d_dev || (n > 1 && !e_dev) || !workspace) return BRDC_STATUS_INVALID_POINTER;
-    const int leaf = leaf_of(opts);
+    const int leaf = leaf_of(opts, n);
     const Layout L = tree_layout(n, leaf);
     if (workspace_bytes < L.total || (reinterpret_cast<uintptr_t>(workspace) % kAlign) != 0)
         return BRDC_STATUS_INVALID_WORKSPACE;
@@ -733,7 +799,7 @@ int brdc_merge_stats(int n, const void* workspace, size_t workspace_bytes, void*
                      const brdc_options* opts, int* out_host, int max_merges) {
     if (n < 0 || !options_ok(opts)) return -BRDC_STATUS_INVALID_VALUE;
     if (!workspace || (!out_host && max_merges > 0)) return -BRDC_STATUS_INVALID_POINTER;
-    const int leaf = leaf_of(opts);
+    const int leaf = leaf_of(opts, n);
     const Layout L = tree_layout(n, leaf);
     if (workspace_bytes < L.total) return -BRDC_STATUS_INVALID_WORKSPACE;
     const int levels = (n > 0) ? tree_levels(n, leaf) : 0;
@@ -846,7 +912,8 @@ int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork
     }
     if (n <= 1) return 0;
 
-    const Layout L = tree_layout(n, kDefaultLeaf);
+    const int leaf = leaf_of(nullptr, n);
+    const Layout L = tree_layout(n, leaf);
     double *dd = nullptr, *de = nullptr;
     void* ws = nullptr;
     int* dinfo = nullptr;
@@ -860,7 +927,7 @@ int brdc_dstebr(int n, double* d, double* e, double* work, int lwork, int* iwork
                hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice) != hipSuccess) {
         result = -1000 - BRDC_STATUS_HIP_ERROR;
     } else {
-        const int st = enqueue_solve(n, dd, de, ws, L, nullptr, dinfo, kDefaultLeaf, Tuning{});
+        const int st = enqueue_solve(n, dd, de, ws, L, nullptr, dinfo, leaf, Tuning{});
         if (st != BRDC_STATUS_SUCCESS ||
             hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
@@ -908,7 +975,7 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
     if (!p) return BRDC_STATUS_ALLOC_FAILED;
     p->nmat = nmat;
     p->rows = (int)rows;
-    p->leaf = leaf_of(opts);
+    p->leaf = leaf_of(opts, sizes, nmat);
     p->tune = tuning_of(opts);
     p->moff.resize((size_t)nmat + 1);
     std::vector<int> lev((size_t)nmat, 0);
@@ -925,37 +992,87 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
         p->poff[b + 1] = p->poff[b] + (long long)tasks * 2 * sizes[b];
         p->rows_max = std::max(p->rows_max, sizes[b]);
     }
-    // leaves of every matrix (a matrix that fits one leaf is solved there)
+    // each matrix's subtrees (the same choice as a single solve): hs[b]
+    // merge levels at the bottom of its tree run in k_subtree, 0 for none
+    // (all of them, or none when the batch holds more subtree roots than
+    // the tuning admits)
+    std::vector<int> hs((size_t)nmat, 0);
+    long long nsub = 0;
+    for (int b = 0; b < nmat; b++) {
+        if (lev[b] < 1) continue;
+        const int sub = subtree_depth(sizes[b], lev[b]);
+        if (sub >= 0) {
+            hs[b] = lev[b] - sub;
+            nsub += 1LL << sub;
+        }
+    }
+    if (nsub > p->tune.sub_max) std::fill(hs.begin(), hs.end(), 0);
+    // leaves of the matrices without subtrees (a matrix that fits one leaf
+    // is solved there)
     for (int b = 0; b < nmat; b++) {
         if (lev[b] < 0) continue;
         for (int i = 0; i < (1 << lev[b]); i++) {
             int s, sz;
             node_geom(sizes[b], lev[b], i, &s, &sz);
-            p->items.push_back(make_int4(p->moff[b] + s, sz, b, lev[b] == 0 ? kItemRoot : 0));
             p->leaf_rows = std::max(p->leaf_rows, sz);
+            p->nleaves++;
+            if (hs[b] == 0)
+                p->items.push_back(make_int4(p->moff[b] + s, sz, b, lev[b] == 0 ? kItemRoot : 0));
         }
     }
-    p->nleaves = (int)p->items.size();
-    // merges h levels above the leaves, h = 1 .. H
+    p->leaf_items = (int)p->items.size();
+    // merges h levels above the leaves, h = 1 .. H: first those that run as
+    // level launches, then the subtrees' merges (records only), by matrix;
+    // inside[h][b]: the position of matrix b's first subtree merge of height h
+    std::vector<std::vector<size_t>> inside((size_t)H + 1, std::vector<size_t>((size_t)nmat, 0));
     for (int h = 1; h <= H; h++) {
         HostLevel L;
         L.plan_at = p->items.size();
         L.counts_rec = p->merge_records;
-        for (int b = 0; b < nmat; b++) {
-            if (lev[b] < h) continue;
-            const int depth = lev[b] - h;
-            for (int i = 0; i < (1 << depth); i++) {
-                int s, sz;
-                node_geom(sizes[b], depth, i, &s, &sz);
-                p->items.push_back(make_int4(p->moff[b] + s, sz, b, depth == 0 ? kItemRoot : 0));
-                L.maxsz = std::max(L.maxsz, sz);
-                L.any_nonroot = L.any_nonroot || depth > 0;
+        for (int pass = 0; pass < 2; pass++) {
+            for (int b = 0; b < nmat; b++) {
+                if (lev[b] < h || (pass == 0) != (h > hs[b])) continue;
+                const int depth = lev[b] - h;
+                if (pass == 1) inside[(size_t)h][(size_t)b] = p->items.size() - L.plan_at;
+                for (int i = 0; i < (1 << depth); i++) {
+                    int s, sz;
+                    node_geom(sizes[b], depth, i, &s, &sz);
+                    p->items.push_back(
+                        make_int4(p->moff[b] + s, sz, b, depth == 0 ? kItemRoot : 0));
+                    if (pass == 0) {
+                        L.maxsz = std::max(L.maxsz, sz);
+                        L.any_nonroot = L.any_nonroot || depth > 0;
+                    }
+                }
             }
+            if (pass == 0) L.launch = (int)(p->items.size() - L.plan_at);
         }
         L.merges = (int)(p->items.size() - L.plan_at);
         p->merge_records += (size_t)L.merges;
         p->levels.push_back(L);
     }
+    // subtree roots: matrix b's nodes at depth lev[b] - hs[b]; the merges of
+    // root q at relative depth r (height hs[b] - r) are records
+    // inside[h][b] + q 2^r onward of height h
+    p->sub_at = p->items.size();
+    for (int b = 0; b < nmat; b++) {
+        if (hs[b] == 0) continue;
+        const int depth = lev[b] - hs[b];
+        for (int q = 0; q < (1 << depth); q++) {
+            int s, sz;
+            node_geom(sizes[b], depth, q, &s, &sz);
+            p->items.push_back(make_int4(p->moff[b] + s, sz, b,
+                                         (depth == 0 ? kItemRoot : 0) | (hs[b] << kSubLevelShift)));
+            for (int r = 0; r < kSubMaxLevels; r++) {
+                const int h = hs[b] - r;
+                p->srec.push_back(r < hs[b] ? (int)(p->levels[(size_t)h - 1].counts_rec +
+                                                    inside[(size_t)h][(size_t)b]) +
+                                                  (q << r)
+                                            : 0);
+            }
+        }
+    }
+    p->nsub = (int)(p->items.size() - p->sub_at);
     // gradient items: matrices by descending order (ties by index) in every
     // launch, so the longest work comes first; a long matrix's tasks split
     // into chains, twist-search ranges, z directions and contribution ranges
@@ -1000,7 +1117,7 @@ int brdc_batch_plan_create(int nmat, const int* sizes, const brdc_options* opts,
                               (long long)(p->tprefix[b + 1] - p->tprefix[b]) * grad_slot(sizes[b]);
     }
     p->layout = make_layout(p->rows, nmat, p->merge_records, std::max<size_t>(p->items.size(), 1),
-                            p->gitems.size());
+                            p->gitems.size(), (size_t)p->nsub);
     *plan = p;
     return BRDC_STATUS_SUCCESS;
 }
@@ -1083,6 +1200,10 @@ int brdc_batch_upload(const brdc_batch_plan* plan, void* workspace, size_t works
         hipMemcpyAsync(at<long long>(workspace, L.cpof), plan->cpoff.data(),
                        plan->cpoff.size() * sizeof(long long), hipMemcpyHostToDevice, st) !=
             hipSuccess ||
+        (!plan->srec.empty() &&
+         hipMemcpyAsync(at<int>(workspace, L.srec), plan->srec.data(),
+                        plan->srec.size() * sizeof(int), hipMemcpyHostToDevice, st) !=
+             hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
         return BRDC_STATUS_HIP_ERROR;
     return BRDC_STATUS_SUCCESS;
@@ -1206,7 +1327,7 @@ extern "C" int brdc_dstebr_device_noalloc(int n, double* d_dev, const double* e_
     if (n < 0 || n > BRDC_NOALLOC_MAX_N || !options_ok(opts)) return BRDC_STATUS_INVALID_VALUE;
     if (n == 0) return BRDC_STATUS_SUCCESS;
     if (!d_dev || (n > 1 && !e_dev)) return BRDC_STATUS_INVALID_POINTER;
-    const int leaf = leaf_of(opts);
+    const int leaf = leaf_of(opts, n);
     const Layout L = tree_layout(n, leaf);
     if (L.total > sizeof(scratch)) return BRDC_STATUS_INVALID_VALUE;
     void* ws = nullptr;

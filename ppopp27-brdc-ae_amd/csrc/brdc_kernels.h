@@ -318,30 +318,19 @@ __host__ __device__ inline size_t leaf_lds_bytes(int maxm) {
     return sizeof(double) * 4 * (size_t)maxm * leaf_lanes(maxm);
 }
 
-// maxm: the largest leaf's row count (sizes the dynamic LDS: the fewer bytes
-// per leaf, the more leaves are resident per CU); LP = leaf_lanes(maxm).
-template <int LP>
-__global__ __launch_bounds__(LP) void k_leaf(
-    Geo g, int nleaves, int maxm, const int* __restrict__ moff, const double* __restrict__ d_in,
-    const double* __restrict__ ew, const double* __restrict__ scal, double* out_final,
-    double* out_d, double* out_lo, double* out_hi, int* info, int* info_user) {
-    extern __shared__ double leaf_lds[];
-    const int lane = threadIdx.x;
-    const int leaf = blockIdx.x * LP + lane;
-    if (leaf >= nleaves) return;
-    // element k of this lane's vectors at [k * LP + lane]
-    double* const vd = leaf_lds + lane;
-    double* const ve = vd + (size_t)maxm * LP;
-    double* const v0 = ve + (size_t)maxm * LP;
-    double* const v1 = v0 + (size_t)maxm * LP;
-#define sd_(k) vd[(k) * LP]
-#define se_(k) ve[(k) * LP]
-#define s0_(k) v0[(k) * LP]
-#define s1_(k) v1[(k) * LP]
-    const Item it = g.item(leaf);
-    const int s = it.s, m = it.sz, n = g.n;
-    const bool single = (it.flags & kItemRoot) != 0;  // the leaf is its whole matrix
-    const double scale = scal[2 * it.mat];
+// One leaf of m rows on a lane's four LDS vectors (element k of each at
+// [k * ls]): loads the scaled diagonal with the cut corrections, the
+// off-diagonal and the unit end rows, then runs the QL iteration.  On return
+// vd, v0, v1 hold the eigenvalues and the two kept rows (unordered); the
+// result is false when the iteration limit was reached.
+__device__ __forceinline__ bool leaf_ql(int s, int m, int n, double scale,
+                                        const double* __restrict__ d_in,
+                                        const double* __restrict__ ew, int ls, double* vd,
+                                        double* ve, double* v0, double* v1) {
+#define sd_(k) vd[(k) * ls]
+#define se_(k) ve[(k) * ls]
+#define s0_(k) v0[(k) * ls]
+#define s1_(k) v1[(k) * ls]
     for (int k = 0; k < m; k++) {
         double dk = d_in[s + k] * scale;
         if (k == 0 && s > 0) dk -= fabs(ew[s - 1]);
@@ -457,35 +446,68 @@ __global__ __launch_bounds__(LP) void k_leaf(
             mm = negligible(gg, dl_new, dnx) ? l : nxt;
         }
     }
-    if (failed) report_item_failure(fail_site(info, moff, n, it), m);
-    // ascending order by rank (ties by position), carrying the two rows
-    for (int a = 0; a < m; a++) {
-        const double x = sd_(a);
-        int rank = 0, b = 0;
-        for (; b + 4 <= m; b += 4) {
-            double y[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) y[u] = sd_(b + u);
-#pragma unroll
-            for (int u = 0; u < 4; u++) rank += (y[u] < x) || (y[u] == x && b + u < a);
-        }
-        for (; b < m; b++) {
-            const double y = sd_(b);
-            rank += (y < x) || (y == x && b < a);
-        }
-        if (single) {
-            out_final[s + rank] = x * scal[2 * it.mat + 1];
-        } else {
-            out_d[s + rank] = x;
-            out_lo[s + rank] = s0_(a);
-            out_hi[s + rank] = s1_(a);
-        }
-    }
-    if (single && info_user) *info_user = *info;  // a one-leaf solve: one matrix
+    return !failed;
 #undef sd_
 #undef se_
 #undef s0_
 #undef s1_
+}
+
+// Ascending order of a solved leaf by rank (ties by position): emit(rank, x,
+// first row, last row) for each of its m eigenvalues.
+template <class Emit>
+__device__ __forceinline__ void leaf_emit(int m, int ls, const double* vd, const double* v0,
+                                          const double* v1, Emit emit) {
+    for (int a = 0; a < m; a++) {
+        const double x = vd[a * ls];
+        int rank = 0, b = 0;
+        for (; b + 4 <= m; b += 4) {
+            double y[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) y[u] = vd[(b + u) * ls];
+#pragma unroll
+            for (int u = 0; u < 4; u++) rank += (y[u] < x) || (y[u] == x && b + u < a);
+        }
+        for (; b < m; b++) {
+            const double y = vd[b * ls];
+            rank += (y < x) || (y == x && b < a);
+        }
+        emit(rank, x, v0[a * ls], v1[a * ls]);
+    }
+}
+
+// maxm: the largest leaf's row count (sizes the dynamic LDS: the fewer bytes
+// per leaf, the more leaves are resident per CU); LP = leaf_lanes(maxm).
+template <int LP>
+__global__ __launch_bounds__(LP) void k_leaf(
+    Geo g, int nleaves, int maxm, const int* __restrict__ moff, const double* __restrict__ d_in,
+    const double* __restrict__ ew, const double* __restrict__ scal, double* out_final,
+    double* out_d, double* out_lo, double* out_hi, int* info, int* info_user) {
+    extern __shared__ double leaf_lds[];
+    const int lane = threadIdx.x;
+    const int leaf = blockIdx.x * LP + lane;
+    if (leaf >= nleaves) return;
+    // element k of this lane's vectors at [k * LP + lane]
+    double* const vd = leaf_lds + lane;
+    double* const ve = vd + (size_t)maxm * LP;
+    double* const v0 = ve + (size_t)maxm * LP;
+    double* const v1 = v0 + (size_t)maxm * LP;
+    const Item it = g.item(leaf);
+    const int s = it.s, m = it.sz, n = g.n;
+    const bool single = (it.flags & kItemRoot) != 0;  // the leaf is its whole matrix
+    if (!leaf_ql(s, m, n, scal[2 * it.mat], d_in, ew, LP, vd, ve, v0, v1))
+        report_item_failure(fail_site(info, moff, n, it), m);
+    const double unscale = scal[2 * it.mat + 1];
+    leaf_emit(m, LP, vd, v0, v1, [&](int rank, double x, double lo, double hi) {
+        if (single) {
+            out_final[s + rank] = x * unscale;
+        } else {
+            out_d[s + rank] = x;
+            out_lo[s + rank] = lo;
+            out_hi[s + rank] = hi;
+        }
+    });
+    if (single && info_user) *info_user = *info;  // a one-leaf solve: one matrix
 }
 
 // ---------------------------------------------------------------------------
@@ -514,17 +536,16 @@ __device__ __forceinline__ int count_less_equal(const double* a, int len, double
     return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void k_merge_prepare(
-    Geo g, int bpm, const double* __restrict__ sd, const double* __restrict__ slo,
-    const double* __restrict__ shi, const double* __restrict__ ew, double* __restrict__ md,
-    double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2) {
-    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    const Item it = g.item(merge);
-    const int s = it.s, sz = it.sz;
-    const int idx = part * kThreads + threadIdx.x;
-    if (idx >= sz) return;
+// Element idx < sz of the merge at rows [s, s + sz), rho the off-diagonal at
+// its cut.
+__device__ __forceinline__ void prepare_element(int s, int sz, int idx, double rho,
+                                                const double* __restrict__ sd,
+                                                const double* __restrict__ slo,
+                                                const double* __restrict__ shi,
+                                                double* __restrict__ md, double* __restrict__ mz,
+                                                double* __restrict__ mq1,
+                                                double* __restrict__ mq2) {
     const int cut = sz / 2;
-    const double rho = ew[s + cut - 1];
     const double t0 = 1.0 / sqrt(2.0);
     const double x = sd[s + idx];
     int rank;
@@ -545,6 +566,18 @@ __global__ __launch_bounds__(kThreads) void k_merge_prepare(
     mz[s + rank] = z * t0;
     mq1[s + rank] = q1;
     mq2[s + rank] = q2;
+}
+
+__global__ __launch_bounds__(kThreads) void k_merge_prepare(
+    Geo g, int bpm, const double* __restrict__ sd, const double* __restrict__ slo,
+    const double* __restrict__ shi, const double* __restrict__ ew, double* __restrict__ md,
+    double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2) {
+    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
+    const Item it = g.item(merge);
+    const int s = it.s, sz = it.sz;
+    const int idx = part * kThreads + threadIdx.x;
+    if (idx >= sz) return;
+    prepare_element(s, sz, idx, ew[s + sz / 2 - 1], sd, slo, shi, md, mz, mq1, mq2);
 }
 
 // ---------------------------------------------------------------------------
@@ -702,27 +735,54 @@ __device__ __forceinline__ int speculative_end(int s, int t, int m, int cap, dou
     return (u < m && steps == cap) ? -2 : u;
 }
 
-// One block per merge; each wave owns a contiguous range of positions (and
-// later of t), walked in coalesced 64-wide tiles with ballot/popcount
-// compaction; cross-wave offsets come from per-wave totals in LDS.
+// Kept (K), Givens-deflated (G) and tiny-deflated (T) counts of a merge.
+struct DeflateCounts {
+    int K, G, T;
+};
+
+// Barrier among the T threads that share a deflation: the block, or for
+// T = 64 one wavefront, whose lanes run in lockstep, so the workgroup-scope
+// fences (the waits for the wave's own loads and stores) and a stop to the
+// compiler's reordering make every lane's writes visible to the others.
 template <int T>
-__global__ __launch_bounds__(T) void k_deflate(
-    Geo g, const double* __restrict__ ew, double* __restrict__ md,
-    double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2, int* __restrict__ P,
+__device__ __forceinline__ void group_sync() {
+    if constexpr (T == kWave) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
+// LDS ints a deflation of T threads needs (deflate_merge's sh argument)
+template <int T>
+constexpr int deflate_sh_ints() {
+    return 2 * (T / kWave) + 1;
+}
+
+// The merge at rows [s, s + sz) deflated by T threads: the block, or one
+// wavefront when T = 64 (every thread of the group calls it); ecut is the
+// off-diagonal at its cut, sh the group's deflate_sh_ints<T>() LDS ints.
+// rec, if not null, is the merge's count record.  Each wave owns a
+// contiguous range of positions (and later of t), walked in coalesced
+// 64-wide tiles with ballot/popcount compaction; cross-wave offsets come
+// from per-wave totals in LDS.
+template <int T>
+__device__ __forceinline__ DeflateCounts deflate_merge(
+    int s, int sz, double ecut, double* __restrict__ md, double* __restrict__ mz,
+    double* __restrict__ mq1, double* __restrict__ mq2, int* __restrict__ P,
     int* __restrict__ rend, int* __restrict__ fl, double* __restrict__ ad, double* __restrict__ az,
     double2* __restrict__ adz, double2* __restrict__ aq, double* __restrict__ gd,
     double* __restrict__ gq1, double* __restrict__ gq2, double* __restrict__ td,
-    double* __restrict__ tq1, double* __restrict__ tq2, int* __restrict__ counts) {
+    double* __restrict__ tq1, double* __restrict__ tq2, int* __restrict__ rec, int* sh) {
     constexpr int NW = T / kWave;
-    __shared__ int wa[NW], wb[NW];
-    __shared__ int s_fix;
-    const int merge = blockIdx.x;
-    const Item it = g.item(merge);
-    const int s = it.s, sz = it.sz;
-    const int cut = sz / 2;
-    const double rho = fabs(2.0 * ew[s + cut - 1]);
+    int* const wa = sh;
+    int* const wb = sh + NW;
+    int& s_fix = sh[2 * NW];
+    const double rho = fabs(2.0 * ecut);
     const double tol = 8.0 * kEps * fmax(fabs(md[s]), fabs(md[s + sz - 1]));
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    const int tid = threadIdx.x & (T - 1), lane = tid & (kWave - 1), w = tid / kWave;
     const unsigned long long lt = (1ull << lane) - 1ull;
 
     // ---- A: tiny-z flags, non-tiny compaction into P[0..m)
@@ -740,7 +800,7 @@ __global__ __launch_bounds__(T) void k_deflate(
     }
     if (lane == 0) wa[w] = cnt;
     if (tid == 0) s_fix = 0;
-    __syncthreads();
+    group_sync<T>();
     int off = 0, m = 0;
     for (int q = 0; q < NW; q++) {
         const int v = wa[q];
@@ -754,11 +814,11 @@ __global__ __launch_bounds__(T) void k_deflate(
         if (keep) P[s + off + __popcll(b & lt)] = p;
         off += __popcll(b);
     }
-    __syncthreads();
+    group_sync<T>();
 
     // ---- B: speculative runs from every rotating pair (independent per t)
     for (int t = 1 + tid; t < m; t += T) rend[s + t] = speculative_end(s, t, m, kSpecCap, tol, P, md, mz);
-    __syncthreads();
+    group_sync<T>();
 
     // ---- B2: disjoint speculative runs make every candidate a real start
     // (the sequential walk would start each of them); any overlap, or a walk
@@ -781,19 +841,19 @@ __global__ __launch_bounds__(T) void k_deflate(
     }
     first = wave_min(first);
     const bool wbad = __any(bad);
-    __syncthreads();  // wa/wb reuse
+    group_sync<T>();  // wa/wb reuse
     if (lane == 0) {
         wa[w] = carry;
         wb[w] = first;
         if (wbad) s_fix = 1;
     }
-    __syncthreads();
+    group_sync<T>();
     if (lane == 0) {
         int cin = INT_MIN;
         for (int q = 0; q < w; q++) cin = max(cin, wa[q]);
         if (wb[w] != INT_MAX && cin >= wb[w]) s_fix = 1;
     }
-    __syncthreads();
+    group_sync<T>();
     if (s_fix) {
         if (tid == 0) {
             int cover = -1;
@@ -811,7 +871,7 @@ __global__ __launch_bounds__(T) void k_deflate(
                 cover = e;
             }
         }
-        __syncthreads();
+        group_sync<T>();
     }
 
     // ---- C: Givens-list offsets (scan of run lengths over t) and replay
@@ -822,9 +882,9 @@ __global__ __launch_bounds__(T) void k_deflate(
         runs += (e >= 0) ? e - t : 0;
     }
     for (int o = 32; o > 0; o >>= 1) runs += __shfl_xor(runs, o, kWave);
-    __syncthreads();  // wa reuse
+    group_sync<T>();  // wa reuse
     if (lane == 0) wa[w] = runs;
-    __syncthreads();
+    group_sync<T>();
     int goff = 0, G = 0;
     for (int q = 0; q < NW; q++) {
         const int v = wa[q];
@@ -839,7 +899,7 @@ __global__ __launch_bounds__(T) void k_deflate(
         if (e >= 0) replay_run(s, t, e, goff + incl - len, tol, P, md, mz, mq1, mq2, fl, gd, gq1, gq2);
         goff += __shfl(incl, kWave - 1, kWave);
     }
-    __syncthreads();
+    group_sync<T>();
 
     // ---- D: compaction into kept poles and tiny-deflated entries
     int na = 0, nt = 0;
@@ -849,12 +909,12 @@ __global__ __launch_bounds__(T) void k_deflate(
         na += __popcll(__ballot(f == 0));
         nt += __popcll(__ballot(f == 2));
     }
-    __syncthreads();  // wa/wb reuse
+    group_sync<T>();  // wa/wb reuse
     if (lane == 0) {
         wa[w] = na;
         wb[w] = nt;
     }
-    __syncthreads();
+    group_sync<T>();
     int aoff = 0, toff = 0, K = 0, Tn = 0;
     for (int q = 0; q < NW; q++) {
         const int va = wa[q], vt = wb[q];
@@ -885,15 +945,33 @@ __global__ __launch_bounds__(T) void k_deflate(
         aoff += __popcll(ba);
         toff += __popcll(bt);
     }
-    if (tid == 0) {
-        counts[kCountStride * merge + 0] = K;
-        counts[kCountStride * merge + 1] = G;
-        counts[kCountStride * merge + 2] = Tn;
-        counts[kCountStride * merge + 3] = 0;
-        counts[kCountStride * merge + 4] = 0;
-        counts[kCountStride * merge + 5] = 0;
-        counts[kCountStride * merge + 6] = 0;
+    if (tid == 0 && rec) {
+        rec[0] = K;
+        rec[1] = G;
+        rec[2] = Tn;
+        rec[3] = 0;
+        rec[4] = 0;
+        rec[5] = 0;
+        rec[6] = 0;
     }
+    group_sync<T>();  // wa/wb are free for the group's next merge
+    return DeflateCounts{K, G, Tn};
+}
+
+// One block per merge.
+template <int T>
+__global__ __launch_bounds__(T) void k_deflate(
+    Geo g, const double* __restrict__ ew, double* __restrict__ md,
+    double* __restrict__ mz, double* __restrict__ mq1, double* __restrict__ mq2, int* __restrict__ P,
+    int* __restrict__ rend, int* __restrict__ fl, double* __restrict__ ad, double* __restrict__ az,
+    double2* __restrict__ adz, double2* __restrict__ aq, double* __restrict__ gd,
+    double* __restrict__ gq1, double* __restrict__ gq2, double* __restrict__ td,
+    double* __restrict__ tq1, double* __restrict__ tq2, int* __restrict__ counts) {
+    __shared__ int sh[deflate_sh_ints<T>()];
+    const int merge = blockIdx.x;
+    const Item it = g.item(merge);
+    deflate_merge<T>(it.s, it.sz, ew[it.s + it.sz / 2 - 1], md, mz, mq1, mq2, P, rend, fl, ad, az,
+                     adz, aq, gd, gq1, gq2, td, tq1, tq2, counts + kCountStride * merge, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -2204,26 +2282,16 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
     }
 }
 
-// Merge step 6: the node's ascending order (DLAMRG of roots with deflated
-// values, dlaed7_br.f:146, and the boundary-row persist, 155-160) as a 3-way
-// merge of three sorted lists: roots, Givens-deflated, tiny-deflated.
-__global__ __launch_bounds__(kThreads) void k_finalize(
-    Geo g, int bpm, const int* __restrict__ counts, const double* __restrict__ rlam,
-    const double* __restrict__ qo1, const double* __restrict__ qo2, const double* __restrict__ gd,
-    const double* __restrict__ gq1, const double* __restrict__ gq2, const double* __restrict__ td,
-    const double* __restrict__ tq1, const double* __restrict__ tq2, double* __restrict__ out_final,
-    double* __restrict__ od, double* __restrict__ olo, double* __restrict__ ohi,
-    const double* __restrict__ scal, const int* __restrict__ info, int* info_user) {
-    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
-    const Item it = g.item(merge);
-    const int s = it.s, sz = it.sz;
-    const bool root = (it.flags & kItemRoot) != 0;
-    // one matrix: the root merge's last kernel hands INFO to the caller
-    if (root && blockIdx.x == 0 && threadIdx.x == 0 && info_user) *info_user = *info;
-    const int idx = part * kThreads + threadIdx.x;
-    if (idx >= sz) return;
-    const int K = counts[kCountStride * merge], G = counts[kCountStride * merge + 1],
-              T = counts[kCountStride * merge + 2];
+// Element idx of a merge at rows from s with K roots, G Givens-deflated and
+// T tiny-deflated values: its rank in the merged order and its entries in the
+// node's state (or, for the root merge of a matrix, the unscaled eigenvalue).
+__device__ __forceinline__ void finalize_element(
+    int s, int idx, int K, int G, int T, bool root, double unscale,
+    const double* __restrict__ rlam, const double* __restrict__ qo1,
+    const double* __restrict__ qo2, const double* __restrict__ gd, const double* __restrict__ gq1,
+    const double* __restrict__ gq2, const double* __restrict__ td, const double* __restrict__ tq1,
+    const double* __restrict__ tq2, double* __restrict__ out_final, double* __restrict__ od,
+    double* __restrict__ olo, double* __restrict__ ohi) {
     double x, q1, q2;
     int rank;
     if (idx < K) {
@@ -2245,12 +2313,35 @@ __global__ __launch_bounds__(kThreads) void k_finalize(
         q2 = tq2[s + t];
     }
     if (root) {
-        out_final[s + rank] = x * scal[2 * it.mat + 1];
+        out_final[s + rank] = x * unscale;
     } else {
         od[s + rank] = x;
         olo[s + rank] = q1;
         ohi[s + rank] = q2;
     }
+}
+
+// Merge step 6: the node's ascending order (DLAMRG of roots with deflated
+// values, dlaed7_br.f:146, and the boundary-row persist, 155-160) as a 3-way
+// merge of three sorted lists: roots, Givens-deflated, tiny-deflated.
+__global__ __launch_bounds__(kThreads) void k_finalize(
+    Geo g, int bpm, const int* __restrict__ counts, const double* __restrict__ rlam,
+    const double* __restrict__ qo1, const double* __restrict__ qo2, const double* __restrict__ gd,
+    const double* __restrict__ gq1, const double* __restrict__ gq2, const double* __restrict__ td,
+    const double* __restrict__ tq1, const double* __restrict__ tq2, double* __restrict__ out_final,
+    double* __restrict__ od, double* __restrict__ olo, double* __restrict__ ohi,
+    const double* __restrict__ scal, const int* __restrict__ info, int* info_user) {
+    const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
+    const Item it = g.item(merge);
+    const int s = it.s, sz = it.sz;
+    const bool root = (it.flags & kItemRoot) != 0;
+    // one matrix: the root merge's last kernel hands INFO to the caller
+    if (root && blockIdx.x == 0 && threadIdx.x == 0 && info_user) *info_user = *info;
+    const int idx = part * kThreads + threadIdx.x;
+    if (idx >= sz) return;
+    const int* rec = counts + kCountStride * merge;
+    finalize_element(s, idx, rec[0], rec[1], rec[2], root, root ? scal[2 * it.mat + 1] : 1.0,
+                     rlam, qo1, qo2, gd, gq1, gq2, td, tq1, tq2, out_final, od, olo, ohi);
 }
 
 }  // namespace brdc

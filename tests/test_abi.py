@@ -39,6 +39,16 @@ def test_profile_classes_match_the_library():
     assert names == camel
     assert int(re.search(r"constexpr int kStreamMinK = (\d+);", src).group(1)) == \
         _lib.STREAM_MIN_K
+    import brdc
+    assert int(re.search(r"constexpr int kDefaultLeaf = (\d+);", src).group(1)) == \
+        brdc._DEFAULT_LEAF
+    assert int(re.search(r"constexpr int kSubtreeLeaf = (\d+);", src).group(1)) == \
+        brdc._SUBTREE_LEAF
+    sub = open(os.path.join(ROOT, "ppopp27-brdc-ae_amd", "csrc", "brdc_subtree.h")).read()
+    assert int(re.search(r"constexpr int kSubRows = (\d+);", sub).group(1)) == _lib.SUBTREE_ROWS
+    assert int(re.search(r"constexpr int kSubMaxLevels = (\d+);", sub).group(1)) == 6
+    assert int(re.search(r"constexpr int kSubMaxLog2 = (\d+);", src).group(1)) == \
+        brdc._SUB_MAX_LOG2
 
 
 def test_status_strings():
@@ -244,10 +254,12 @@ def test_tuning_options_are_validated_without_a_gpu():
     for kw in ({"stream_min_k": 256}, {"stream_min_k": 1 << 20}, {"secular_lanes_log2": 12},
                {"secular_lanes_log2": 24}, {"wide_max_waves": 1}, {"wide_max_waves": 4},
                {"leaf_max": 24, "stream_min_k": 1024, "secular_lanes_log2": 17,
-                "wide_max_waves": 2}):
+                "wide_max_waves": 2}, {"subtree_max_log2": -1}, {"subtree_max_log2": 1},
+               {"subtree_max_log2": 30}):
         assert status(**kw) == 0, kw
     for kw in ({"stream_min_k": 100}, {"stream_min_k": -1}, {"secular_lanes_log2": 11},
-               {"secular_lanes_log2": 25}, {"wide_max_waves": 3}, {"wide_max_waves": 16}):
+               {"secular_lanes_log2": 25}, {"wide_max_waves": 3}, {"wide_max_waves": 16},
+               {"subtree_max_log2": -2}, {"subtree_max_log2": 31}):
         assert status(**kw) == 3, kw
     o = _lib.options()
     o.reserved[2] = 1

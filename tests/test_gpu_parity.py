@@ -293,7 +293,11 @@ def test_harness_gpu_columns(gpu):
         assert np.all(st["k"] + st["givens"] + st["tiny"] == st["size"])
         big = st["k"] > 2
         assert np.all(st["evals"][big] >= st["k"][big])
-        assert set(r["level_ms"]) == set(range(-1, int(st["level"].max()) + 1))
+        # levels of nodes over 64 rows run as level launches; the subtree
+        # launch (level -1) holds the rest
+        launched = {int(x) for x in st["level"][st["size"] > 64]}
+        assert set(r["level_ms"]) == {-1} | launched
+        assert "subtree" in r["level_ms"][-1]
     assert "iter_max" in out.getvalue()
 
 

@@ -83,7 +83,7 @@ def test_secular_launch_slots_cover_every_merge(sec):
 @pytest.mark.parametrize("kind", ["normal", "uniform", "clustered", "toeplitz"])
 def test_device_secular_on_every_merge_of_a_solve(sec, orc, kind):
     """Every secular problem the oracle's tree meets on a seeded n = 3000
-    matrix (leaf 16, the device default): the device root finder converges
+    matrix (leaf 16): the device root finder converges
     on each root and agrees with the oracle's DLAED4."""
     from brdc.generators import generate, seed_for
     n = 3000
