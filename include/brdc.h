@@ -45,7 +45,7 @@ typedef struct brdc_options {
     int wide_max_waves;     /* widest group (waves) of the product-form and
                                boundary-row kernels, 1, 2, 4 or 8; 0 selects
                                the default (8) */
-    int subtree_max_log2;   /* the bottom of the tree (nodes of at most 64 rows
+    int subtree_max_log2;   /* the bottom of the tree (nodes of at most 128 rows
                                with all their leaves and merges) runs in one
                                launch, a block per node, when there are at most
                                2^subtree_max_log2 such nodes (a batch: over all
@@ -162,7 +162,7 @@ int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void 
  * norm, prep_e, leaf, prepare, deflate, secular (merges below 2048 poles),
  * wprod, rows, finalize, secular_stream (merges of 2048 poles or more, every
  * evaluation after the first), secular_mid (their first, midpoint
- * evaluation), subtree (the leaves and every merge of nodes of at most 64
+ * evaluation), subtree (the leaves and every merge of nodes of at most 128
  * rows, in one launch) (BRDC_KERNEL_CLASSES of them), the summed
  * milliseconds and launch counts.
  */
@@ -174,7 +174,7 @@ int brdc_profile_end(double *ms_out, int *launches_out, int nclasses);
  * TridiagMergeLevelDiagnostics): ms_out and launches_out hold (nlevels + 1)
  * rows of nclasses entries; row 0 gathers the launches before the merges
  * (norm, prep_e, leaf, subtree: the subtree launch holds the levels of nodes
- * of at most 64 rows, which then have no row of their own), row 1 + l the
+ * of at most 128 rows, which then have no row of their own), row 1 + l the
  * merges of level l (tree depth, 0 = the root merge; for a batch, forest
  * height l, 1 = one level above the leaves).  Levels beyond nlevels are
  * dropped. */

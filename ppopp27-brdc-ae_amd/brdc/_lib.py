@@ -39,7 +39,7 @@ NOALLOC_BYTES = 20 << 20
 # (kStreamMinK in csrc/brdc.hip)
 STREAM_MIN_K = 2048
 # largest node (rows) solved inside the subtree kernel, with all of its leaves
-SUBTREE_ROWS = 64
+SUBTREE_ROWS = 128
 
 
 class Options(ctypes.Structure):

@@ -11,7 +11,7 @@ tile/launch knobs of brdc_options.
   which the lanes per root follow (more lanes: shorter passes, more waves);
 * wide_max_waves: the widest group of the product-form and boundary-row
   kernels (waves splitting one 64-item group's stream);
-* subtree_max_log2: up to how many subtree roots (nodes of at most 64 rows)
+* subtree_max_log2: up to how many subtree roots (nodes of at most 128 rows)
   the bottom of the trees runs in the one-launch subtree kernel (latency
   bound: it wins while the subtrees are few, the level launches win when
   thousands of subtrees fill the device anyway); -1 never, 30 always.

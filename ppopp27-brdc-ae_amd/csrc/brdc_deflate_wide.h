@@ -26,7 +26,7 @@ namespace brdc {
 
 constexpr int kWideChunk = 4096;    // positions (or poles) per block
 constexpr int kWideThreads = 1024;  // threads per block, 4 positions each
-constexpr int kWideMaxMerges = 8;   // levels with at most this many merges ...
+constexpr int kWideMaxMerges = 4;   // levels with at most this many merges ...
 constexpr int kWideMinSize = 4096;  // ... of at least this many rows use it
 constexpr int kWideFields = 8;      // ints per chunk record
 

@@ -1,31 +1,36 @@
-// brdc_subtree.h -- the bottom of the merge tree, one wavefront per subtree.
+// brdc_subtree.h -- the bottom of the merge tree, one block per subtree.
 //
 // The deepest merge levels of a tree hold thousands of merges of a few dozen
 // rows each.  As separate launches, each of their six merge steps costs a
 // kernel boundary (L2 written back, every operand fetched again) for a few
-// microseconds of work.  Here one wavefront takes a whole subtree of at most
+// microseconds of work.  Here one block takes a whole subtree of at most
 // kSubRows rows -- its leaves and every merge below its root, the root
 // included -- and keeps the node state and all merge scratch in LDS: the
 // leaves' QL iterations, then per level the merge-path prepare, the
-// deflation (the same block routine as k_deflate, with one wave), the
-// secular roots packed one per lane across the level's merges, the product
+// deflation (the same block routine as k_deflate, one wave per merge), the
+// secular roots packed two lanes each across the level's merges, the product
 // form, the boundary rows and the three-way finalize.  Only the subtree
 // root's state (or, when the subtree is a whole matrix, its eigenvalues)
 // goes back to HBM, with each merge's count record.
 //
 // The arithmetic is that of the level kernels (dlaed0_br.f:95-210 through
-// dlaed9_br.f:181-331, as cited there) with one lane per root and per pole,
-// so the results agree with the level-by-level path to rounding.
+// dlaed9_br.f:181-331, as cited there) with two lanes per root and one per
+// pole, so the results agree with the level-by-level path to rounding.
 #pragma once
 
 #include "brdc_kernels.h"
 
 namespace brdc {
 
-constexpr int kSubRows = 64;         // rows of a subtree: one lane per row
-constexpr int kSubMerges = kSubRows / 2;
+// Wavefronts of a subtree block and the largest subtree: two lanes per
+// secular root, so 32 rows per wave.
+constexpr int kSubWaves = 4;
+constexpr int kSubRows = 128;
+static_assert(kSubRows == 32 * kSubWaves, "two lanes per root of the subtree's largest merge");
+constexpr int kSubThreads = kSubWaves * kWave;
+constexpr int kSubMaxLevels = 6;     // merge levels of a subtree (leaves of >= 2 rows)
+constexpr int kSubMerges = 1 << (kSubMaxLevels - 1);  // merges of one level
 constexpr int kSubLeafCol = 2 * kSubRows;  // leaf column doubles: (largest leaf) x leaves
-constexpr int kSubMaxLevels = 6;     // merge levels of a subtree (leaves of >= 1 row)
 
 // Subtree roots of one launch.  A single matrix: the nodes at depth g.depth,
 // with `levels` the tree's leaf depth; merge (r, i) of subtree b (relative
@@ -155,17 +160,14 @@ __device__ __forceinline__ void sub_rows(const SubtreeLds& L, int ms, int K, int
     *q2o = acc2 / nrm;
 }
 
-// Threads of a subtree block: two wavefronts.  The secular roots of a level
-// (at most kSubRows) go two lanes each, half to each wave, and small merges
-// deflate one per wave; a SIMD then holds two such waves, which hide part of
-// each other's dependent FP64 chains.
-constexpr int kSubWaves = 2;
-constexpr int kSubThreads = kSubWaves * kWave;
-
+// The secular roots of a level (at most kSubRows) go two lanes each, dealt
+// round the block's waves, and small merges deflate one per wave; a SIMD
+// holds two such waves, which hide part of each other's dependent FP64
+// chains.
 // Item (a pole or root) of thread tid in the phases that carry FP64 work:
-// consecutive items alternate between the two waves.
+// consecutive items go to consecutive waves.
 __device__ __forceinline__ int sub_item_of(int tid) {
-    return ((tid & (kWave - 1)) << 1) | (tid >> 6);
+    return (tid & (kWave - 1)) * kSubWaves + (tid >> 6);
 }
 
 // One block per subtree root (blockIdx.x).  out_final: the eigenvalues when
@@ -251,10 +253,10 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(
         }
         __syncthreads();
         const int total = L.roff[nm];
-        // ---- secular roots: two lanes each, even roots in wave 0, odd in wave 1
+        // ---- secular roots: two lanes each, consecutive roots in consecutive waves
         {
             constexpr int SL = 2;
-            const int j = ((lane >> 1) << 1) | wave, c = lane & 1;
+            const int j = (lane >> 1) * kSubWaves + wave, c = lane & 1;
             int m = 0;
             while (m + 1 < nm && L.roff[m + 1] <= j) m++;
             const bool has = j < total;

@@ -274,6 +274,7 @@ def test_shipped_reduced_dense_input(gpu):
 
 def test_harness_gpu_columns(gpu):
     import io
+    import brdc
     from brdc import harness
     out = io.StringIO()
     rows = harness.main(["--sizes=300,5000", "--data=normal,toeplitz,clustered,splitmix-normal",
@@ -293,9 +294,9 @@ def test_harness_gpu_columns(gpu):
         assert np.all(st["k"] + st["givens"] + st["tiny"] == st["size"])
         big = st["k"] > 2
         assert np.all(st["evals"][big] >= st["k"][big])
-        # levels of nodes over 64 rows run as level launches; the subtree
-        # launch (level -1) holds the rest
-        launched = {int(x) for x in st["level"][st["size"] > 64]}
+        # levels of larger nodes run as level launches; the subtree launch
+        # (level -1) holds the rest
+        launched = {int(x) for x in st["level"][st["size"] > brdc._lib.SUBTREE_ROWS]}
         assert set(r["level_ms"]) == {-1} | launched
         assert "subtree" in r["level_ms"][-1]
     assert "iter_max" in out.getvalue()
