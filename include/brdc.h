@@ -160,11 +160,13 @@ int brdc_merge_stats(int n, const void *workspace, size_t workspace_bytes, void 
  * bracketed by two HIP events on its own stream.  end synchronises those
  * events and writes, for each of the first nclasses classes in the order
  * norm, prep_e, leaf, prepare, deflate, secular (merges below 2048 poles),
- * wprod, rows, finalize, secular_stream (merges of 2048 poles or more, every
- * evaluation after the first), secular_mid (their first, midpoint
- * evaluation), subtree (the leaves and every merge of nodes of at most 128
- * rows, in one launch) (BRDC_KERNEL_CLASSES of them), the summed
- * milliseconds and launch counts.
+ * wprod and rows (the product form and boundary rows of merges of more than
+ * 128 poles at levels of nodes of more than 256 rows), finalize (which
+ * computes them for the other merges), secular_stream (merges of 2048
+ * poles or more, every evaluation after the first), secular_mid (their
+ * first, midpoint evaluation), subtree (the leaves and every merge of nodes
+ * of at most 128 rows, in one launch) (BRDC_KERNEL_CLASSES of them), the
+ * summed milliseconds and launch counts.
  */
 #define BRDC_KERNEL_CLASSES 12
 int brdc_profile_begin(void);

@@ -299,6 +299,19 @@ def in_subtree(solver, stats) -> np.ndarray:
     return (hs[0] > 0) & (stats["level"] >= lev[0] - hs[0])
 
 
+def _level_max_size(solver, stats) -> np.ndarray:
+    """Per merge record, the largest merge of its level launch (the maxsz the
+    driver sizes that level's launches by): the largest record of the same
+    level outside the subtree kernel."""
+    sub = in_subtree(solver, stats)
+    out = np.zeros(len(stats), dtype=np.int64)
+    for lv in np.unique(stats["level"]):
+        m = (stats["level"] == lv) & ~sub
+        if m.any():
+            out[stats["level"] == lv] = int(stats["size"][m].max())
+    return out
+
+
 def _root_merges(solver, stats) -> np.ndarray:
     """Mask of the records that are a matrix's root merge (no boundary rows)."""
     if isinstance(solver, BatchSolver):
@@ -321,8 +334,10 @@ def profile_kernels(solver: Solver, problems, stream=None):
     STREAM_MIN_K); secular_stream for the others' evaluations after the
     first), 4 per pole for the first, midpoint evaluation of the merges of
     the threshold or more (secular_mid: the sum alone), 3 per pole pair
-    (product-form w, non-root merges), 9 per pole pair (boundary rows,
-    non-root merges).
+    (product-form w) and 9 per pole pair (boundary rows) of the non-root
+    merges that k_wprod and k_rows solve (more than SMALL_MERGE_TOP poles, at
+    levels of nodes of more than SMALL_MERGE rows; the finalize launch solves
+    the others).
     """
     import torch
     L = _lib.load()
@@ -357,8 +372,15 @@ def profile_kernels(solver: Solver, problems, stream=None):
         flops["secular_mid"] += float(np.sum(4.0 * k[big] * k[big]))
         root = _root_merges(solver, stats)
         nonroot = ~root & (stats["k"] > 2)
-        flops["wprod"] += float(np.sum(3.0 * k[nonroot & ~sub] ** 2))
-        flops["rows"] += float(np.sum(9.0 * k[nonroot & ~sub] ** 2))
+        # merges of at most SMALL_MERGE poles at levels of nodes of at most
+        # SMALL_MERGE rows, else of at most SMALL_MERGE_TOP poles, get their
+        # product form and rows in the finalize launch (an HBM class, whose
+        # time then includes them); the larger ones in wprod and rows
+        kmax = np.where(_level_max_size(solver, stats) <= _lib.SMALL_MERGE, _lib.SMALL_MERGE,
+                        _lib.SMALL_MERGE_TOP)
+        wide = stats["k"] > kmax
+        flops["wprod"] += float(np.sum(3.0 * k[nonroot & ~sub & wide] ** 2))
+        flops["rows"] += float(np.sum(9.0 * k[nonroot & ~sub & wide] ** 2))
         flops["subtree"] += float(np.sum(7.0 * k[sub] * ev[sub]) +
                                   np.sum(12.0 * k[nonroot & sub] ** 2))
         # every launched merge moves its rows once; the subtree kernel reads

@@ -38,6 +38,12 @@ NOALLOC_BYTES = 20 << 20
 # merges of at least this many poles go to the streamed secular kernel
 # (kStreamMinK in csrc/brdc.hip)
 STREAM_MIN_K = 2048
+# merges of at most SMALL_MERGE poles (at levels of nodes of at most
+# SMALL_MERGE rows; SMALL_MERGE_TOP above) get their product form and
+# boundary rows in the finalize launch, one block each (kSmallMaxK,
+# kSmallTopK in csrc/brdc_kernels.h)
+SMALL_MERGE = 256
+SMALL_MERGE_TOP = 128
 # largest node (rows) solved inside the subtree kernel, with all of its leaves
 SUBTREE_ROWS = 128
 

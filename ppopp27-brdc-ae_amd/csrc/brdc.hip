@@ -58,6 +58,9 @@ struct Tuning {
     long long sub_max = 1LL << kSubMaxLog2;  // most subtree blocks (0: never)
 };
 constexpr int kMidMaxGroups = 1024; // midpoint-pass grid cap (tiles of 64 roots)
+// finalize grids of at most this many blocks compute the small merges' rows
+// themselves (about one block per CU)
+constexpr long long kFinFuseMaxBlocks = 256;
 
 // Workgroups of the streamed secular kernel that fit the device at once
 // (CUs x resident groups per CU), cached per device.
@@ -268,13 +271,14 @@ struct ProfScope {
 
 // the O(K^2) kernels come in widths of 1..16 waves per 64-item group
 template <class K1, class K2, class K4, class K8, class... Args>
-void launch_wide(K1 k1, K2 k2, K4 k4, K8 k8, int WL, int blocks, hipStream_t st, Args... args) {
+void launch_wide(K1 k1, K2 k2, K4 k4, K8 k8, int WL, int blocks, size_t lds, hipStream_t st,
+                 Args... args) {
     const dim3 grid(blocks);
     switch (WL) {
-        case 1: hipLaunchKernelGGL(k1, grid, dim3(kGroup), 0, st, args...); break;
-        case 2: hipLaunchKernelGGL(k2, grid, dim3(2 * kGroup), 0, st, args...); break;
-        case 4: hipLaunchKernelGGL(k4, grid, dim3(4 * kGroup), 0, st, args...); break;
-        default: hipLaunchKernelGGL(k8, grid, dim3(8 * kGroup), 0, st, args...); break;
+        case 1: hipLaunchKernelGGL(k1, grid, dim3(kGroup), lds, st, args...); break;
+        case 2: hipLaunchKernelGGL(k2, grid, dim3(2 * kGroup), lds, st, args...); break;
+        case 4: hipLaunchKernelGGL(k4, grid, dim3(4 * kGroup), lds, st, args...); break;
+        default: hipLaunchKernelGGL(k8, grid, dim3(8 * kGroup), lds, st, args...); break;
     }
 }
 
@@ -524,29 +528,50 @@ int enqueue(const Schedule& S, double* d, const double* e, void* ws, const Layou
                                rs, rn1, rn2, counts, S.moff, info);
             BRDC_LAUNCH_CHECK();
         }
-        if (lv.any_nonroot) {
+        // merges of at most ksmall poles get their product form and rows in
+        // the finalize launch; the larger ones here
+        const int ksmall = small_max_k(maxsz);
+        if (lv.any_nonroot && maxsz > kSmallMaxK) {
             {
                 ProfScope ps(kWprod, st);
-                launch_wide(k_wprod<1>, k_wprod<2>, k_wprod<4>, k_wprod<8>, WL,
-                            merges * bpg, st, g, bpg, (const int*)counts, (const double*)ad,
+                launch_wide(k_wprod<1>, k_wprod<2>, k_wprod<4>, k_wprod<8>, WL, merges * bpg, 0,
+                            st, g, bpg, ksmall, (const int*)counts, (const double*)ad,
                             (const double*)az, (const double2*)rs, (const double*)rn1,
                             (const double*)rn2, pw);
             }
             BRDC_LAUNCH_CHECK();
             {
                 ProfScope ps(kRows, st);
-                launch_wide(k_rows<1>, k_rows<2>, k_rows<4>, k_rows<8>, WL,
-                            merges * bpg, st, g, bpg, (const int*)counts, (const double2*)aq,
+                launch_wide(k_rows<1>, k_rows<2>, k_rows<4>, k_rows<8>, WL, merges * bpg, 0, st,
+                            g, bpg, ksmall, (const int*)counts, (const double2*)aq,
                             (const double2*)pw, (const double2*)rs, (const double*)rn1,
                             (const double*)rn2, qo1, qo2);
             }
             BRDC_LAUNCH_CHECK();
         }
+        // the small merges' rows inside the finalize launch while its grid is
+        // small (a single matrix); a launch of their own before a many-block
+        // finalize (a batch), where the fused kernel's registers would cost
+        // the finalize blocks their occupancy
+        const bool fused = lv.any_nonroot && (long long)merges * bpm <= kFinFuseMaxBlocks;
+        const size_t small_lds = small_merge_lds(std::min(maxsz, ksmall));
+        if (lv.any_nonroot && !fused) {
+            ProfScope ps(kFinalize, st);
+            hipLaunchKernelGGL(k_small_merge, dim3(merges),
+                               dim3(small_merge_threads(maxsz, merges)), small_lds, st, g,
+                               ksmall, (const int*)counts, (const double*)ad, (const double*)az,
+                               (const double2*)rs, (const double*)rn1, (const double*)rn2,
+                               (const double2*)aq, qo1, qo2);
+            BRDC_LAUNCH_CHECK();
+        }
         {
             ProfScope ps(kFinalize, st);
-            hipLaunchKernelGGL(k_finalize, grid, dim3(kThreads), 0, st, g, bpm, counts, rlam, qo1,
-                               qo2, gd, gq1, gq2, td, tq1, tq2, d, sd[nxt], slo[nxt], shi[nxt],
-                               (const double*)scal, (const int*)info, info_user);
+            hipLaunchKernelGGL(fused ? k_finalize<true> : k_finalize<false>, grid, dim3(kThreads),
+                               fused ? small_lds : 0, st, g, bpm, fused ? ksmall : 0, counts, rlam,
+                               qo1, qo2, gd, gq1, gq2, td, tq1, tq2, d, sd[nxt], slo[nxt],
+                               shi[nxt], (const double*)scal, (const int*)info, info_user,
+                               (const double*)ad, (const double*)az, (const double2*)rs,
+                               (const double*)rn1, (const double*)rn2, (const double2*)aq);
         }
         BRDC_LAUNCH_CHECK();
         cur = nxt;

@@ -2049,33 +2049,61 @@ void k_secular_stream(
     }
 }
 
-// Merge step 4: product-form update vector (dlaed9_br.f:218-266): one lane per
-// pole i streaming the roots j: w_i = sign(z_i) sqrt(-prod_j delta_j(i) /
-// prod_{j != i} (d_i - d_j)).  delta_j(i) comes from root j's compact state;
-// the two pole-adjacent entries are root j's near1/near2 (dlaed9_br.f:272-331).
-template <int WL>
-__global__ __launch_bounds__(kGroup * WL) void k_wprod(
-    Geo geo, int bpg, const int* __restrict__ counts, const double* __restrict__ ad,
-    const double* __restrict__ az, const double2* __restrict__ rs,
-    const double* __restrict__ rn1, const double* __restrict__ rn2, double2* __restrict__ pw) {
-    __shared__ double red[WL > 1 ? WL : 1][kGroup];
-    const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
-    const int K = uniform(counts[kCountStride * merge]);
-    const int g0 = g * kGroup;
-    if (K <= 2 || g0 >= K) return;
-    const Item it = geo.item(merge);
-    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
-    const int s = it.s;
-    const int lane = threadIdx.x & (kGroup - 1);
-    const int c = uniform(threadIdx.x / kGroup);
-    const int W = waves_for(K, WL);
-    if (W == 1 && c > 0) return;
-    const int j0 = (c < W) ? (int)(((long long)K * c) / W) : K;
-    const int j1 = (c < W) ? (int)(((long long)K * (c + 1)) / W) : K;
-    const int i = min(g0 + lane, K - 1);
-    const CDoubles d = as_const(ad + s);
-    const CPairs rst = as_const(rs + s);
-    const double di = ad[s + i];
+// Merges of at most small_max_k(maxsz) poles get their w and boundary rows
+// from the first block of their finalize launch, in LDS.  Levels of merges
+// of at most kSmallMaxK rows need no other launch for them; above that the
+// product-form and boundary-row launches take the merges of more than
+// kSmallTopK poles.
+constexpr int kSmallMaxK = 256;
+constexpr int kSmallTopK = 128;
+__host__ __device__ inline int small_max_k(int maxsz) {
+    return maxsz <= kSmallMaxK ? kSmallMaxK : kSmallTopK;
+}
+
+// Root data of the product-form loop: (d_org, tau) pairs and d.  RootsSmem
+// reads them through scalar loads (one base per batch, so a batch arrives in a
+// few wide loads); RootsLds reads a merge staged in LDS.
+struct RootsSmem {
+    cdbl rs, d;
+    static constexpr bool kLazy = false;
+    __device__ __forceinline__ void batch(int j, double2 (&r)[kBatch], double (&dj)[kBatch]) const {
+        const cdbl rb = rs + 2 * (size_t)j;
+        const cdbl db = d + j;
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            r[b] = make_double2(rb[2 * b], rb[2 * b + 1]);
+            dj[b] = db[b];
+        }
+    }
+    __device__ __forceinline__ double2 r(int j) const {
+        return make_double2(rs[2 * (size_t)j], rs[2 * (size_t)j + 1]);
+    }
+    __device__ __forceinline__ double dj(int j) const { return d[j]; }
+};
+struct RootsLds {
+    const double2* rs;
+    const double* d;
+    static constexpr bool kLazy = false;
+    __device__ __forceinline__ void batch(int j, double2 (&r)[kBatch], double (&dj)[kBatch]) const {
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            r[b] = rs[j + b];
+            dj[b] = d[j + b];
+        }
+    }
+    __device__ __forceinline__ double2 r(int j) const { return rs[j]; }
+    __device__ __forceinline__ double dj(int j) const { return d[j]; }
+};
+
+// Merge step 4: product-form update vector (dlaed9_br.f:218-266) at pole i
+// (group of poles from g0), over the roots [j0, j1) of a K-pole merge:
+// the share prod_j delta_j(i) / prod_{j != i} (d_i - d_j) of
+// w_i = sign(z_i) sqrt(-prod_j delta_j(i) / prod_{j != i} (d_i - d_j)).
+// delta_j(i) comes from root j's compact state; the two pole-adjacent
+// entries are root j's near1/near2 (n1, n2; dlaed9_br.f:272-331).
+template <class R>
+__device__ __forceinline__ double wprod_share(const R& roots, const double* n1, const double* n2,
+                                              int i, double di, int g0, int j0, int j1, int K) {
     // the product of ratios as a ratio of products num / den * 2^ex, with the
     // exponents moved out after every batch: no division per pair
     double num = 1.0, den = 1.0;
@@ -2092,8 +2120,8 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     auto term = [&](double2 r, double dj, int j, bool check) {
         double dl = (di - r.x) - r.y;
         if (check) {  // j == i and the pole-adjacent roots lie in the window only
-            if (i == j) dl = rn1[s + j];
-            else if (i == j + 1) dl = rn2[s + j];
+            if (i == j) dl = n1[j];
+            else if (i == j + 1) dl = n2[j];
             num *= dl;
             den *= (j == i) ? 1.0 : di - dj;
         } else {
@@ -2104,34 +2132,31 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
     auto range = [&](int ja, int jb, bool check) {
         int j = ja;
         for (; j + kBatch <= jb; j += kBatch) {
-            // one base per batch, so the batch arrives in a few wide scalar loads
-            const cdbl rb = rst.p + 2 * (size_t)j;
-            const cdbl db = d.p + j;
             double2 r[kBatch];
             double dj[kBatch];
-#pragma unroll
-            for (int b = 0; b < kBatch; b++) {
-                r[b] = make_double2(rb[2 * b], rb[2 * b + 1]);
-                dj[b] = db[b];
-            }
+            roots.batch(j, r, dj);
             // even and odd pairs in two product chains, joined before the
             // exponents move out
             const double n0 = num, d0 = den;
             num = 1.0;
             den = 1.0;
+            auto tb = [&](int b) {
+                if constexpr (R::kLazy) term(roots.r(j + b), roots.dj(j + b), j + b, check);
+                else term(r[b], dj[b], j + b, check);
+            };
 #pragma unroll
-            for (int b = 0; b < kBatch; b += 2) term(r[b], dj[b], j + b, check);
-            const double n1 = num, d1 = den;
+            for (int b = 0; b < kBatch; b += 2) tb(b);
+            const double n1v = num, d1v = den;
             num = 1.0;
             den = 1.0;
 #pragma unroll
-            for (int b = 1; b < kBatch; b += 2) term(r[b], dj[b], j + b, check);
-            num = (n0 * n1) * num;
-            den = (d0 * d1) * den;
+            for (int b = 1; b < kBatch; b += 2) tb(b);
+            num = (n0 * n1v) * num;
+            den = (d0 * d1v) * den;
             renorm();
         }
         for (; j < jb; j++) {
-            term(rst[j], d[j], j, check);
+            term(roots.r(j), roots.dj(j), j, check);
             renorm();
         }
     };
@@ -2145,69 +2170,63 @@ __global__ __launch_bounds__(kGroup * WL) void k_wprod(
         range(j0, jend, false);
     }
     if (jlast >= 0) {  // root K-1: near1 at pole K-2, near2 at pole K-1
-        const double2 r = rs[s + jlast];
+        const double2 r = roots.r(jlast);
         double dl = (di - r.x) - r.y;
-        if (i == K - 2) dl = rn1[s + jlast];
-        else if (i == K - 1) dl = rn2[s + jlast];
+        if (i == K - 2) dl = n1[jlast];
+        else if (i == K - 1) dl = n2[jlast];
         num *= dl;
-        den *= (jlast == i) ? 1.0 : di - ad[s + jlast];
+        den *= (jlast == i) ? 1.0 : di - roots.dj(jlast);
         renorm();
     }
-    double prod = __builtin_amdgcn_ldexp(num / den, ex);  // this wave's share of the product
-    if (W > 1) {
-        red[c][lane] = prod;
-        __syncthreads();
-        prod = 1.0;
-        for (int cc = 0; cc < W; cc++) prod *= red[cc][lane];
-    }
-    if (c == 0 && g0 + lane < K) pw[s + i] = make_double2(di, copysign(sqrt(-prod), az[s + i]));
+    return __builtin_amdgcn_ldexp(num / den, ex);
 }
 
-// Merge step 5: the two boundary rows of the merged eigenvector block
-// (dlaed9_br.f:272-331): one lane per root j streaming the poles i,
-// x_i = w_i / delta_j(i), row_r(j) = sum_i q_r(i) x_i / ||x||.
-template <int WL>
-__global__ __launch_bounds__(kGroup * WL) void k_rows(
-    Geo geo, int bpg, const int* __restrict__ counts, const double2* __restrict__ aq,
-    const double2* __restrict__ pw, const double2* __restrict__ rs,
-    const double* __restrict__ rn1, const double* __restrict__ rn2, double* __restrict__ qo1,
-    double* __restrict__ qo2) {
-    __shared__ double red[WL > 1 ? WL : 1][3][kGroup];
-    const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
-    const int K = uniform(counts[kCountStride * merge]);
-    const int g0 = g * kGroup;
-    if (g0 >= K) return;
-    const Item it = geo.item(merge);
-    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
-    const int s = it.s;
-    const int lane = threadIdx.x & (kGroup - 1);
-    const int c = uniform(threadIdx.x / kGroup);
-    if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172)
-        const int j = g0 + lane;
-        if (c != 0 || j >= K) return;
-        const double v0 = rn1[s + j], v1 = rn2[s + j];
-        const double2 q0 = aq[s];
-        double a1 = q0.x * v0, a2 = q0.y * v0;
-        if (K == 2) {
-            const double2 q1 = aq[s + 1];
-            a1 += q1.x * v1;
-            a2 += q1.y * v1;
+// Pole data of the boundary-row loop: product-form pairs (d_i, w_i) and the
+// merged children's rows (q1_i, q2_i), through scalar loads or from LDS.
+struct PolesSmem {
+    cdbl pw, aq;
+    static constexpr bool kLazyQ = false;
+    __device__ __forceinline__ void batch(int i, double2 (&p)[kBatch], double2 (&q)[kBatch]) const {
+        const cdbl pb = pw + 2 * (size_t)i;
+        const cdbl qb = aq + 2 * (size_t)i;
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            p[b] = make_double2(pb[2 * b], pb[2 * b + 1]);
+            q[b] = make_double2(qb[2 * b], qb[2 * b + 1]);
         }
-        qo1[s + j] = a1;
-        qo2[s + j] = a2;
-        return;
     }
-    const int W = waves_for(K, WL);
-    if (W == 1 && c > 0) return;
-    const int i0 = (c < W) ? (int)(((long long)K * c) / W) : K;
-    const int i1 = (c < W) ? (int)(((long long)K * (c + 1)) / W) : K;
-    const int j = min(g0 + lane, K - 1);
-    const double2 r = rs[s + j];
-    const double dorg = r.x, tau = r.y, nr1 = rn1[s + j], nr2 = rn2[s + j];
+    __device__ __forceinline__ double2 p(int i) const {
+        return make_double2(pw[2 * (size_t)i], pw[2 * (size_t)i + 1]);
+    }
+    __device__ __forceinline__ double2 q(int i) const {
+        return make_double2(aq[2 * (size_t)i], aq[2 * (size_t)i + 1]);
+    }
+};
+struct PolesLds {
+    const double2* pw;
+    const double2* aq;
+    static constexpr bool kLazyQ = false;
+    __device__ __forceinline__ void batch(int i, double2 (&p)[kBatch], double2 (&q)[kBatch]) const {
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            p[b] = pw[i + b];
+            q[b] = aq[i + b];
+        }
+    }
+    __device__ __forceinline__ double2 p(int i) const { return pw[i]; }
+    __device__ __forceinline__ double2 q(int i) const { return aq[i]; }
+};
+
+// Merge step 5: the two boundary rows of the merged eigenvector block
+// (dlaed9_br.f:272-331) at root j (group of roots from g0), over the poles
+// [i0, i1) of a K-pole merge, K >= 3: x_i = w_i / delta_j(i) summed into
+// temp = sum x_i^2, acc_r = sum_i q_r(i) x_i (row_r(j) = acc_r / sqrt(temp)).
+template <class P>
+__device__ __forceinline__ void rows_share(const P& poles, int j, double2 r, double nr1,
+                                           double nr2, int g0, int i0, int i1, int K,
+                                           double& temp, double& acc1, double& acc2) {
+    const double dorg = r.x, tau = r.y;
     const int a = (j == K - 1) ? K - 2 : j;
-    const CPairs P = as_const(pw + s);
-    const CPairs Q = as_const(aq + s);
-    double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
     auto delta = [&](double2 p, int i, bool check) {
         double dl = (p.x - dorg) - tau;
         if (check) {
@@ -2224,15 +2243,8 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
     auto range = [&](int ia, int ib, bool check) {
         int i = ia;
         for (; i + kBatch <= ib; i += kBatch) {
-            // one base per batch, so the batch arrives in a few wide scalar loads
-            const cdbl pb = P.p + 2 * (size_t)i;
-            const cdbl qb = Q.p + 2 * (size_t)i;
             double2 p[kBatch], q[kBatch];
-#pragma unroll
-            for (int b = 0; b < kBatch; b++) {
-                p[b] = make_double2(pb[2 * b], pb[2 * b + 1]);
-                q[b] = make_double2(qb[2 * b], qb[2 * b + 1]);
-            }
+            poles.batch(i, p, q);
 #pragma unroll
             for (int b = 0; b < kBatch; b += 4) {
                 // one reciprocal for four poles, as in the streamed secular
@@ -2243,13 +2255,17 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
                 const double x01 = dl[0] * dl[1], x23 = dl[2] * dl[3];
                 const double rq = rcp_stream(x01 * x23);
                 const double r01 = x23 * rq, r23 = x01 * rq;
-                accumulate(p[b].y * (dl[1] * r01), q[b]);
-                accumulate(p[b + 1].y * (dl[0] * r01), q[b + 1]);
-                accumulate(p[b + 2].y * (dl[3] * r23), q[b + 2]);
-                accumulate(p[b + 3].y * (dl[2] * r23), q[b + 3]);
+                auto qv = [&](int u) {
+                    if constexpr (P::kLazyQ) return poles.q(i + b + u);
+                    else return q[b + u];
+                };
+                accumulate(p[b].y * (dl[1] * r01), qv(0));
+                accumulate(p[b + 1].y * (dl[0] * r01), qv(1));
+                accumulate(p[b + 2].y * (dl[3] * r23), qv(2));
+                accumulate(p[b + 3].y * (dl[2] * r23), qv(3));
             }
         }
-        for (; i < ib; i++) accumulate(P[i].y * rcp_stream(delta(P[i], i, check)), Q[i]);
+        for (; i < ib; i++) accumulate(poles.p(i).y * rcp_stream(delta(poles.p(i), i, check)), poles.q(i));
     };
     // poles that may be pole-adjacent to a root of this group: [g0, g0+64]
     // and the last root's K-2, K-1
@@ -2263,6 +2279,161 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
         range(i0, tail, false);
     }
     range(tail, i1, true);
+}
+
+// Boundary rows of a merge of K <= 2 poles: the secular eigenvector (near1,
+// near2 of each root) applied to the merged rows (dlaed9_br.f:146-172).
+__device__ __forceinline__ void rows_direct(int K, double v0, double v1, double2 q0, double2 q1,
+                                            double* o1, double* o2) {
+    double a1 = q0.x * v0, a2 = q0.y * v0;
+    if (K == 2) {
+        a1 += q1.x * v1;
+        a2 += q1.y * v1;
+    }
+    *o1 = a1;
+    *o2 = a2;
+}
+
+// LDS of a small merge solved whole by one block: doubles per pole.
+constexpr int kSmallMergeDoubles = 12;  // ad, az, rn1, rn2, rs (2), aq (2), pw (2), rows (2)
+__host__ __device__ constexpr size_t small_merge_lds(int maxsz) {
+    return sizeof(double) * kSmallMergeDoubles * (size_t)(maxsz < kSmallMaxK ? maxsz : kSmallMaxK);
+}
+// A merge of K <= kSmallMaxK poles at rows from s, its product form and
+// boundary rows computed by the whole block (nt threads) into o1[j], o2[j]
+// (LDS, j < K): poles and root states staged in LDS (sh), then w and the
+// rows with L lanes per pole (root), L from K and nt: lane c of an item
+// takes the c-th of L consecutive root (pole) ranges, with the arithmetic of
+// the wide kernels' shares, and the L shares meet in an xor butterfly (w:
+// products of the shares; rows: sums).  K <= 2 takes the secular
+// eigenvector directly.  Every thread of the block calls it.
+__device__ __forceinline__ void small_merge_rows(int s, int K, int nt, double* sh,
+                                                 const double* __restrict__ ad,
+                                                 const double* __restrict__ az,
+                                                 const double2* __restrict__ rs,
+                                                 const double* __restrict__ rn1,
+                                                 const double* __restrict__ rn2,
+                                                 const double2* __restrict__ aq, double* o1,
+                                                 double* o2) {
+    const int tid = threadIdx.x;
+    if (K <= 2) {
+        if (tid < K)
+            rows_direct(K, rn1[s + tid], rn2[s + tid], aq[s], K == 2 ? aq[s + 1] : aq[s],
+                        &o1[tid], &o2[tid]);
+        __syncthreads();
+        return;
+    }
+    double* const sd = sh;
+    double* const sz = sd + K;
+    double* const s1 = sz + K;
+    double* const s2 = s1 + K;
+    double2* const srs = reinterpret_cast<double2*>(s2 + K);
+    double2* const saq = srs + K;
+    double2* const spw = saq + K;
+    for (int x = tid; x < K; x += nt) {
+        sd[x] = ad[s + x];
+        sz[x] = az[s + x];
+        s1[x] = rn1[s + x];
+        s2[x] = rn2[s + x];
+        srs[x] = rs[s + x];
+        saq[x] = aq[s + x];
+    }
+    __syncthreads();
+    // lanes per item: the most (up to 16) that leave each item a lane group
+    int L = 1;
+    while (L < 16 && 2 * L * K <= nt) L <<= 1;
+    const int c = tid & (L - 1), per = nt / L;
+    const int ra = (int)(((long long)K * c) / L), rb = (int)(((long long)K * (c + 1)) / L);
+    const RootsLds roots{srs, sd};
+    for (int base = 0; base < K; base += per) {  // the same trip count in every lane
+        const int i = min(base + tid / L, K - 1);
+        double share = wprod_share(roots, s1, s2, i, sd[i], i & ~(kGroup - 1), ra, rb, K);
+        for (int o = 1; o < L; o <<= 1) share *= __shfl_xor(share, o);
+        if (c == 0 && base + tid / L < K) spw[i] = make_double2(sd[i], copysign(sqrt(-share), sz[i]));
+    }
+    __syncthreads();
+    const PolesLds poles{spw, saq};
+    for (int base = 0; base < K; base += per) {
+        const int j = min(base + tid / L, K - 1);
+        double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
+        rows_share(poles, j, srs[j], s1[j], s2[j], j & ~(kGroup - 1), ra, rb, K, temp, acc1, acc2);
+        for (int o = 1; o < L; o <<= 1) {
+            temp += __shfl_xor(temp, o);
+            acc1 += __shfl_xor(acc1, o);
+            acc2 += __shfl_xor(acc2, o);
+        }
+        if (c == 0 && base + tid / L < K) {
+            const double nrm = sqrt(temp);
+            o1[j] = acc1 / nrm;
+            o2[j] = acc2 / nrm;
+        }
+    }
+    __syncthreads();
+}
+
+// Merge step 4 launch (merges of more than ksmall poles): one lane per
+// pole i streaming the roots j, WL waves splitting the root stream (LDS
+// reduce).
+template <int WL>
+__global__ __launch_bounds__(kGroup * WL) void k_wprod(
+    Geo geo, int bpg, int ksmall, const int* __restrict__ counts, const double* __restrict__ ad,
+    const double* __restrict__ az, const double2* __restrict__ rs,
+    const double* __restrict__ rn1, const double* __restrict__ rn2, double2* __restrict__ pw) {
+    __shared__ double red[WL > 1 ? WL : 1][kGroup];
+    const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
+    const int K = uniform(counts[kCountStride * merge]);
+    const int g0 = g * kGroup;
+    if (K <= ksmall || g0 >= K) return;
+    const Item it = geo.item(merge);
+    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
+    const int s = it.s;
+    const int lane = threadIdx.x & (kGroup - 1);
+    const int c = uniform(threadIdx.x / kGroup);
+    const int W = waves_for(K, WL);
+    if (W == 1 && c > 0) return;
+    const int j0 = (c < W) ? (int)(((long long)K * c) / W) : K;
+    const int j1 = (c < W) ? (int)(((long long)K * (c + 1)) / W) : K;
+    const int i = min(g0 + lane, K - 1);
+    const RootsSmem roots{(cdbl) reinterpret_cast<const double*>(rs + s), (cdbl)(ad + s)};
+    const double di = ad[s + i];
+    double prod = wprod_share(roots, rn1 + s, rn2 + s, i, di, g0, j0, j1, K);  // this wave's share
+    if (W > 1) {
+        red[c][lane] = prod;
+        __syncthreads();
+        prod = 1.0;
+        for (int cc = 0; cc < W; cc++) prod *= red[cc][lane];
+    }
+    if (c == 0 && g0 + lane < K) pw[s + i] = make_double2(di, copysign(sqrt(-prod), az[s + i]));
+}
+
+// Merge step 5 launch (merges of more than ksmall poles): one lane per root j
+// streaming the poles i, x_i = w_i / delta_j(i), row_r(j) = sum_i q_r(i) x_i
+// / ||x||, WL waves splitting the pole stream (LDS reduce).
+template <int WL>
+__global__ __launch_bounds__(kGroup * WL) void k_rows(
+    Geo geo, int bpg, int ksmall, const int* __restrict__ counts, const double2* __restrict__ aq,
+    const double2* __restrict__ pw, const double2* __restrict__ rs,
+    const double* __restrict__ rn1, const double* __restrict__ rn2, double* __restrict__ qo1,
+    double* __restrict__ qo2) {
+    __shared__ double red[WL > 1 ? WL : 1][3][kGroup];
+    const int merge = blockIdx.x / bpg, g = blockIdx.x % bpg;
+    const int K = uniform(counts[kCountStride * merge]);
+    const int g0 = g * kGroup;
+    if (K <= ksmall || g0 >= K) return;
+    const Item it = geo.item(merge);
+    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
+    const int s = it.s;
+    const int lane = threadIdx.x & (kGroup - 1);
+    const int c = uniform(threadIdx.x / kGroup);
+    const int W = waves_for(K, WL);
+    if (W == 1 && c > 0) return;
+    const int i0 = (c < W) ? (int)(((long long)K * c) / W) : K;
+    const int i1 = (c < W) ? (int)(((long long)K * (c + 1)) / W) : K;
+    const int j = min(g0 + lane, K - 1);
+    const PolesSmem poles{(cdbl) reinterpret_cast<const double*>(pw + s),
+                          (cdbl) reinterpret_cast<const double*>(aq + s)};
+    double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
+    rows_share(poles, j, rs[s + j], rn1[s + j], rn2[s + j], g0, i0, i1, K, temp, acc1, acc2);
     if (W > 1) {
         red[c][0][lane] = temp;
         red[c][1][lane] = acc1;
@@ -2285,10 +2456,10 @@ __global__ __launch_bounds__(kGroup * WL) void k_rows(
 // Element idx of a merge at rows from s with K roots, G Givens-deflated and
 // T tiny-deflated values: its rank in the merged order and its entries in the
 // node's state (or, for the root merge of a matrix, the unscaled eigenvalue).
+// qo1/qo2: the roots' boundary rows, root i at [i].
 __device__ __forceinline__ void finalize_element(
     int s, int idx, int K, int G, int T, bool root, double unscale,
-    const double* __restrict__ rlam, const double* __restrict__ qo1,
-    const double* __restrict__ qo2, const double* __restrict__ gd, const double* __restrict__ gq1,
+    const double* __restrict__ rlam, const double* qo1, const double* qo2, const double* __restrict__ gd, const double* __restrict__ gq1,
     const double* __restrict__ gq2, const double* __restrict__ td, const double* __restrict__ tq1,
     const double* __restrict__ tq2, double* __restrict__ out_final, double* __restrict__ od,
     double* __restrict__ olo, double* __restrict__ ohi) {
@@ -2297,8 +2468,8 @@ __device__ __forceinline__ void finalize_element(
     if (idx < K) {
         x = rlam[s + idx];
         rank = idx + count_less(gd + s, G, x) + count_less(td + s, T, x);
-        q1 = root ? 0.0 : qo1[s + idx];
-        q2 = root ? 0.0 : qo2[s + idx];
+        q1 = root ? 0.0 : qo1[idx];
+        q2 = root ? 0.0 : qo2[idx];
     } else if (idx < K + G) {
         const int g = idx - K;
         x = gd[s + g];
@@ -2323,25 +2494,78 @@ __device__ __forceinline__ void finalize_element(
 
 // Merge step 6: the node's ascending order (DLAMRG of roots with deflated
 // values, dlaed7_br.f:146, and the boundary-row persist, 155-160) as a 3-way
-// merge of three sorted lists: roots, Givens-deflated, tiny-deflated.
+// merge of three sorted lists: roots, Givens-deflated, tiny-deflated.  With
+// kSmall, a non-root merge of at most ksmall poles has its product form and
+// boundary rows (merge steps 4 and 5) computed first by its first block,
+// whose elements include every root (dynamic LDS: small_merge_lds of the
+// level's largest such merge).
+template <bool kSmall>
 __global__ __launch_bounds__(kThreads) void k_finalize(
-    Geo g, int bpm, const int* __restrict__ counts, const double* __restrict__ rlam,
+    Geo g, int bpm, int ksmall, const int* __restrict__ counts, const double* __restrict__ rlam,
     const double* __restrict__ qo1, const double* __restrict__ qo2, const double* __restrict__ gd,
     const double* __restrict__ gq1, const double* __restrict__ gq2, const double* __restrict__ td,
     const double* __restrict__ tq1, const double* __restrict__ tq2, double* __restrict__ out_final,
     double* __restrict__ od, double* __restrict__ olo, double* __restrict__ ohi,
-    const double* __restrict__ scal, const int* __restrict__ info, int* info_user) {
+    const double* __restrict__ scal, const int* __restrict__ info, int* info_user,
+    const double* __restrict__ ad, const double* __restrict__ az, const double2* __restrict__ rs,
+    const double* __restrict__ rn1, const double* __restrict__ rn2,
+    const double2* __restrict__ aq) {
+    extern __shared__ double fin_small[];
     const int merge = blockIdx.x / bpm, part = blockIdx.x % bpm;
     const Item it = g.item(merge);
     const int s = it.s, sz = it.sz;
     const bool root = (it.flags & kItemRoot) != 0;
     // one matrix: the root merge's last kernel hands INFO to the caller
     if (root && blockIdx.x == 0 && threadIdx.x == 0 && info_user) *info_user = *info;
+    const int* rec = counts + kCountStride * merge;
+    const int K = uniform(rec[0]);
+    const double* q1r = qo1 + s;
+    const double* q2r = qo2 + s;
+    if constexpr (kSmall) {
+        if (!root && part == 0 && K > 0 && K <= ksmall) {  // the same in every thread of the block
+            double* const o1 = fin_small + (size_t)(kSmallMergeDoubles - 2) * K;
+            double* const o2 = o1 + K;
+            small_merge_rows(s, K, kThreads, fin_small, ad, az, rs, rn1, rn2, aq, o1, o2);
+            q1r = o1;
+            q2r = o2;
+        }
+    }
     const int idx = part * kThreads + threadIdx.x;
     if (idx >= sz) return;
-    const int* rec = counts + kCountStride * merge;
-    finalize_element(s, idx, rec[0], rec[1], rec[2], root, root ? scal[2 * it.mat + 1] : 1.0,
-                     rlam, qo1, qo2, gd, gq1, gq2, td, tq1, tq2, out_final, od, olo, ohi);
+    finalize_element(s, idx, K, rec[1], rec[2], root, root ? scal[2 * it.mat + 1] : 1.0,
+                     rlam, q1r, q2r, gd, gq1, gq2, td, tq1, tq2, out_final, od, olo, ohi);
+}
+
+// The same product form and boundary rows for levels whose finalize grid is
+// large (many merges: the separate launch keeps the finalize kernel's
+// registers few): one block of small_merge_threads per merge of 0 < K <=
+// kmax poles, rows to HBM (dynamic LDS: small_merge_lds of the level's
+// largest such merge).
+__host__ __device__ inline int small_merge_threads(int maxsz, int merges) {
+    const int k = maxsz < kSmallMaxK ? maxsz : kSmallMaxK;
+    int p = 64;
+    while (p < k) p <<= 1;
+    const int t = (merges >= 512) ? p : 4 * p;
+    return t > 1024 ? 1024 : t;
+}
+__global__ __launch_bounds__(1024) void k_small_merge(
+    Geo geo, int kmax, const int* __restrict__ counts, const double* __restrict__ ad,
+    const double* __restrict__ az, const double2* __restrict__ rs,
+    const double* __restrict__ rn1, const double* __restrict__ rn2,
+    const double2* __restrict__ aq, double* __restrict__ qo1, double* __restrict__ qo2) {
+    extern __shared__ double sm_small[];
+    const int merge = blockIdx.x;
+    const int K = uniform(counts[kCountStride * merge]);
+    if (K > kmax || K == 0) return;
+    const Item it = geo.item(merge);
+    if (it.flags & kItemRoot) return;  // a root merge keeps no boundary rows
+    double* const o1 = sm_small + (size_t)(kSmallMergeDoubles - 2) * K;
+    double* const o2 = o1 + K;
+    small_merge_rows(it.s, K, blockDim.x, sm_small, ad, az, rs, rn1, rn2, aq, o1, o2);
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        qo1[it.s + j] = o1[j];
+        qo2[it.s + j] = o2[j];
+    }
 }
 
 }  // namespace brdc

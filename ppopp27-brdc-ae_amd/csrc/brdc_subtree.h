@@ -356,9 +356,9 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(
             const int fm = sub_locate(S, r, tid, &fs, &fsz);
             const int idx = tid - fs;
             if (!top) {
-                finalize_element(fs, idx, L.K[fm], L.G[fm], L.T[fm], false, 1.0, L.rlam, L.qo1,
-                                 L.qo2, L.gd, L.gq1, L.gq2, L.td, L.tq1, L.tq2, nullptr, L.sd,
-                                 L.slo, L.shi);
+                finalize_element(fs, idx, L.K[fm], L.G[fm], L.T[fm], false, 1.0, L.rlam,
+                                 L.qo1 + fs, L.qo2 + fs, L.gd, L.gq1, L.gq2, L.td, L.tq1, L.tq2,
+                                 nullptr, L.sd, L.slo, L.shi);
             } else if (mroot) {  // fs == 0: the outputs start at the subtree's first row
                 finalize_element(0, idx, L.K[0], L.G[0], L.T[0], true, scal[2 * it.mat + 1],
                                  L.rlam, L.qo1, L.qo2, L.gd, L.gq1, L.gq2, L.td, L.tq1, L.tq2,

@@ -39,6 +39,11 @@ def test_profile_classes_match_the_library():
     assert names == camel
     assert int(re.search(r"constexpr int kStreamMinK = (\d+);", src).group(1)) == \
         _lib.STREAM_MIN_K
+    kern = open(os.path.join(ROOT, "ppopp27-brdc-ae_amd", "csrc", "brdc_kernels.h")).read()
+    assert int(re.search(r"constexpr int kSmallMaxK = (\d+);", kern).group(1)) == \
+        _lib.SMALL_MERGE
+    assert int(re.search(r"constexpr int kSmallTopK = (\d+);", kern).group(1)) == \
+        _lib.SMALL_MERGE_TOP
     import brdc
     assert int(re.search(r"constexpr int kDefaultLeaf = (\d+);", src).group(1)) == \
         brdc._DEFAULT_LEAF
