@@ -1896,9 +1896,15 @@ void k_secular_stream(
         const long long K = uniform(counts[kCountStride * m]);
         if (K >= kstream) wsum += K * K;
     }
+    // workgroup b takes interval v = 2 (b mod h) + b div h, h = ceil(G/2): the
+    // SIMDs favour older waves and a CU's two workgroups are either b and
+    // b + h (the younger) or neighbours, so every merge gets as many of the
+    // younger workgroups as of the older ones, and no merge lags behind
     const long long G = gridDim.x;
-    const long long wa = wsum * blockIdx.x / G;
-    const long long wb = wsum * (blockIdx.x + 1) / G;
+    const long long h = (G + 1) / 2;
+    const long long v = 2 * ((long long)blockIdx.x % h) + (long long)blockIdx.x / h;
+    const long long wa = wsum * v / G;
+    const long long wb = wsum * (v + 1) / G;
     long long wbase = 0;
     for (int merge = 0; merge < merges; merge++) {
         const int K = uniform(counts[kCountStride * merge]);
