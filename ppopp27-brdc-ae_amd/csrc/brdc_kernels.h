@@ -2072,6 +2072,19 @@ __host__ __device__ inline int small_max_k(int maxsz) {
 struct RootsSmem {
     cdbl rs, d;
     static constexpr bool kLazy = false;
+    // two 4-root buffers with ordered scalar waits measured slower here than
+    // the plain 8-root batch at all but the largest merges (a half's 20
+    // VALU operations hide little of a scalar load)
+    static constexpr bool kPipe = false;
+    __device__ __forceinline__ void batch4(int j, double2 (&r)[4], double (&dj)[4]) const {
+        const cdbl rb = rs + 2 * (size_t)j;
+        const cdbl db = d + j;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            r[b] = make_double2(rb[2 * b], rb[2 * b + 1]);
+            dj[b] = db[b];
+        }
+    }
     __device__ __forceinline__ void batch(int j, double2 (&r)[kBatch], double (&dj)[kBatch]) const {
         const cdbl rb = rs + 2 * (size_t)j;
         const cdbl db = d + j;
@@ -2090,6 +2103,8 @@ struct RootsLds {
     const double2* rs;
     const double* d;
     static constexpr bool kLazy = false;
+    static constexpr bool kPipe = false;
+    __device__ __forceinline__ void batch4(int, double2 (&)[4], double (&)[4]) const {}
     __device__ __forceinline__ void batch(int j, double2 (&r)[kBatch], double (&dj)[kBatch]) const {
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
@@ -2123,20 +2138,57 @@ __device__ __forceinline__ double wprod_share(const R& roots, const double* n1, 
     // roots whose near1/near2 apply to a pole of this group: j in [g0-1, g0+63]
     // (near1 at pole j, near2 at pole j+1) and the last root (poles K-2, K-1)
     const int wa = max(j0, g0 - 1), wb = min(j1, g0 + kGroup);
-    auto term = [&](double2 r, double dj, int j, bool check) {
+    auto term_to = [&](double& nu, double& de, double2 r, double dj, int j, bool check) {
         double dl = (di - r.x) - r.y;
         if (check) {  // j == i and the pole-adjacent roots lie in the window only
             if (i == j) dl = n1[j];
             else if (i == j + 1) dl = n2[j];
-            num *= dl;
-            den *= (j == i) ? 1.0 : di - dj;
+            nu *= dl;
+            de *= (j == i) ? 1.0 : di - dj;
         } else {
-            num *= dl;
-            den *= di - dj;
+            nu *= dl;
+            de *= di - dj;
         }
     };
+    auto term = [&](double2 r, double dj, int j, bool check) { term_to(num, den, r, dj, j, check); };
     auto range = [&](int ja, int jb, bool check) {
         int j = ja;
+        if constexpr (R::kPipe) {
+            if (j + kBatch <= jb) {
+                // a batch of eight roots in two buffers of four: a buffer's
+                // (scalar) loads are waited for before the other's are issued
+                // (scalar loads return out of order: any wait is a wait for
+                // all of them), so each half's arithmetic overlaps the next
+                // half's loads; the product chains are those of the plain
+                // batch (even pairs b = 0, 2, 4, 6; odd pairs 1, 3, 5, 7)
+                double2 rA[4], rB[4];
+                double dA[4], dB[4];
+                roots.batch4(j, rA, dA);
+                for (; j + kBatch <= jb; j += kBatch) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): A is in
+                    roots.batch4(j + 4, rB, dB);
+                    __builtin_amdgcn_sched_barrier(0);   // B in flight before A's work
+                    double ne = 1.0, de = 1.0, no = 1.0, dd = 1.0;
+                    term_to(ne, de, rA[0], dA[0], j, check);
+                    term_to(ne, de, rA[2], dA[2], j + 2, check);
+                    term_to(no, dd, rA[1], dA[1], j + 1, check);
+                    term_to(no, dd, rA[3], dA[3], j + 3, check);
+                    __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // B is in
+                    // the next batch's first half (in range; used if j + 16 <= jb)
+                    roots.batch4(min(j + kBatch, jb - 4), rA, dA);
+                    __builtin_amdgcn_sched_barrier(0);
+                    term_to(ne, de, rB[0], dB[0], j + 4, check);
+                    term_to(ne, de, rB[2], dB[2], j + 6, check);
+                    term_to(no, dd, rB[1], dB[1], j + 5, check);
+                    term_to(no, dd, rB[3], dB[3], j + 7, check);
+                    num = (num * ne) * no;
+                    den = (den * de) * dd;
+                    renorm();
+                }
+            }
+        }
         for (; j + kBatch <= jb; j += kBatch) {
             double2 r[kBatch];
             double dj[kBatch];
@@ -2192,6 +2244,16 @@ __device__ __forceinline__ double wprod_share(const R& roots, const double* n1, 
 struct PolesSmem {
     cdbl pw, aq;
     static constexpr bool kLazyQ = false;
+    static constexpr bool kPipe = true;  // two 4-pole buffers, ordered scalar waits
+    __device__ __forceinline__ void batch4(int i, double2 (&p)[4], double2 (&q)[4]) const {
+        const cdbl pb = pw + 2 * (size_t)i;
+        const cdbl qb = aq + 2 * (size_t)i;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            p[b] = make_double2(pb[2 * b], pb[2 * b + 1]);
+            q[b] = make_double2(qb[2 * b], qb[2 * b + 1]);
+        }
+    }
     __device__ __forceinline__ void batch(int i, double2 (&p)[kBatch], double2 (&q)[kBatch]) const {
         const cdbl pb = pw + 2 * (size_t)i;
         const cdbl qb = aq + 2 * (size_t)i;
@@ -2212,6 +2274,8 @@ struct PolesLds {
     const double2* pw;
     const double2* aq;
     static constexpr bool kLazyQ = false;
+    static constexpr bool kPipe = false;
+    __device__ __forceinline__ void batch4(int, double2 (&)[4], double2 (&)[4]) const {}
     __device__ __forceinline__ void batch(int i, double2 (&p)[kBatch], double2 (&q)[kBatch]) const {
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
@@ -2246,8 +2310,41 @@ __device__ __forceinline__ void rows_share(const P& poles, int j, double2 r, dou
         acc1 = fma(q.x, x, acc1);
         acc2 = fma(q.y, x, acc2);
     };
+    // one group of four poles (one shared reciprocal) from i
+    auto group4 = [&](const double2 (&p)[4], const double2 (&q)[4], int i, bool check) {
+        double dl[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) dl[u] = delta(p[u], i + u, check);
+        const double x01 = dl[0] * dl[1], x23 = dl[2] * dl[3];
+        const double rq = rcp_stream(x01 * x23);
+        const double r01 = x23 * rq, r23 = x01 * rq;
+        accumulate(p[0].y * (dl[1] * r01), q[0]);
+        accumulate(p[1].y * (dl[0] * r01), q[1]);
+        accumulate(p[2].y * (dl[3] * r23), q[2]);
+        accumulate(p[3].y * (dl[2] * r23), q[3]);
+    };
     auto range = [&](int ia, int ib, bool check) {
         int i = ia;
+        if constexpr (P::kPipe) {
+            if (i + kBatch <= ib) {
+                // two buffers of four poles in turn, as in wprod_share: each
+                // group's arithmetic overlaps the next group's scalar loads
+                double2 pA[4], qA[4], pB[4], qB[4];
+                poles.batch4(i, pA, qA);
+                for (; i + kBatch <= ib; i += kBatch) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): A is in
+                    poles.batch4(i + 4, pB, qB);
+                    __builtin_amdgcn_sched_barrier(0);
+                    group4(pA, qA, i, check);
+                    __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // B is in
+                    poles.batch4(min(i + kBatch, ib - 4), pA, qA);  // used if i + 16 <= ib
+                    __builtin_amdgcn_sched_barrier(0);
+                    group4(pB, qB, i + 4, check);
+                }
+            }
+        }
         for (; i + kBatch <= ib; i += kBatch) {
             double2 p[kBatch], q[kBatch];
             poles.batch(i, p, q);
