@@ -41,12 +41,40 @@ struct WideGeo {
     __device__ int* at(int merge, int c) const { return rec + ((size_t)merge * chunks + c) * kWideFields; }
 };
 
-// kept poles of a merge: the sum over its chunks (only chunks that start
-// inside the merge were written)
+// chunks that start inside a merge of sz rows (only their records are written)
+__device__ __forceinline__ int wide_chunks_in(const WideGeo& wg, int sz) {
+    return min(wg.chunks, (sz + kWideChunk - 1) / kWideChunk);
+}
+
+// Field `field` of a merge's chunk records q < n summed, in total and over
+// q < c.  Lane l reads record q0 + l, 64 records to a load (every wave of
+// the block alike): a loop over the records would wait a memory round trip
+// per record.
+struct WideSums {
+    int below, total;
+};
+__device__ __forceinline__ WideSums wide_field_sums(const WideGeo& wg, int merge, int n, int c,
+                                                    int field) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int below = 0, total = 0;
+    for (int q0 = 0; q0 < n; q0 += kWave) {
+        const int q = q0 + lane;
+        int vt = (q < n) ? wg.at(merge, q)[field] : 0;
+        int vb = (q < c) ? vt : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            vt += __shfl_xor(vt, o, kWave);
+            vb += __shfl_xor(vb, o, kWave);
+        }
+        total += vt;
+        below += vb;
+    }
+    return WideSums{__builtin_amdgcn_readfirstlane(below), __builtin_amdgcn_readfirstlane(total)};
+}
+
+// kept poles of a merge: the sum over its chunks
 __device__ __forceinline__ int wide_kept(const WideGeo& wg, int merge, int sz) {
-    int m = 0;
-    for (int q = 0; q < wg.chunks && q * kWideChunk < sz; q++) m += wg.at(merge, q)[kWKeep];
-    return m;
+    return wide_field_sums(wg, merge, wide_chunks_in(wg, sz), 0, kWKeep).total;
 }
 
 // block-wide exclusive scan of one int per thread (kWideThreads threads);
@@ -107,8 +135,7 @@ __global__ __launch_bounds__(kWideThreads) void k_wide_compact(WideGeo wg, const
     const int s = it.s, sz = it.sz;
     const int p0 = c * kWideChunk;
     if (p0 >= sz) return;
-    int base = 0;
-    for (int q = 0; q < c; q++) base += wg.at(merge, q)[kWKeep];
+    const int base = wide_field_sums(wg, merge, c, c, kWKeep).below;
     // each thread owns 4 consecutive positions, so the chunk keeps its order
     constexpr int per = kWideChunk / kWideThreads;
     const int pa = p0 + (int)threadIdx.x * per;
@@ -210,15 +237,32 @@ __global__ __launch_bounds__(kWideThreads) void k_wide_resolve(WideGeo wg, const
     const int s = it.s;
     const int m = wide_kept(wg, merge, it.sz);
     const int nc = (m + kWideChunk - 1) / kWideChunk;  // chunks of poles
-    if (threadIdx.x == 0) {
-        int fix = 0, cin = INT_MIN;
-        for (int q = 0; q < nc; q++) {
-            const int* r = wg.at(merge, q);
-            if (r[kWBad] || (r[kWFirst] != INT_MAX && cin >= r[kWFirst])) fix = 1;
-            cin = max(cin, r[kWCarry]);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x < kWave) {
+        // a chunk needs the sequential resolution if a walk hit the cap or a
+        // run of an earlier chunk reaches its first run start (lane l reads
+        // record q0 + l; cin: the largest run end of the chunks before)
+        bool fix = false;
+        int cin = INT_MIN;
+        for (int q0 = 0; q0 < nc; q0 += kWave) {
+            const int q = q0 + lane;
+            int bad = 0, first = INT_MAX, carry = INT_MIN;
+            if (q < nc) {
+                const int* r = wg.at(merge, q);
+                bad = r[kWBad];
+                first = r[kWFirst];
+                carry = r[kWCarry];
+            }
+            const int incl = wave_incl_max(carry);
+            int excl = __shfl_up(incl, 1, kWave);
+            if (lane == 0) excl = INT_MIN;
+            excl = max(excl, cin);
+            if (bad || (first != INT_MAX && excl >= first)) fix = true;
+            cin = max(cin, __shfl(incl, kWave - 1, kWave));
         }
-        s_fix = fix;
-        if (fix) {  // k_deflate's sequential resolution
+        fix = __any(fix);
+        if (lane == 0) s_fix = fix ? 1 : 0;
+        if (fix && lane == 0) {  // k_deflate's sequential resolution
             const double tol = wg.tol[merge];
             int cover = -1;
             for (int t = 1; t < m; t++) {
@@ -250,14 +294,17 @@ __global__ __launch_bounds__(kWideThreads) void k_wide_resolve(WideGeo wg, const
             __syncthreads();
         }
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kWave) {
+        // Givens offsets: the exclusive prefix sum of the chunks' run lengths
         int g = 0;
-        for (int q = 0; q < nc; q++) {
-            int* r = wg.at(merge, q);
-            r[kWGoff] = g;
-            g += r[kWRuns];
+        for (int q0 = 0; q0 < nc; q0 += kWave) {
+            const int q = q0 + lane;
+            const int v = (q < nc) ? wg.at(merge, q)[kWRuns] : 0;
+            const int incl = wave_incl_sum(v);
+            if (q < nc) wg.at(merge, q)[kWGoff] = g + incl - v;
+            g += __shfl(incl, kWave - 1, kWave);
         }
-        counts[kCountStride * merge + 1] = g;
+        if (lane == 0) counts[kCountStride * merge + 1] = g;
     }
 }
 
@@ -333,17 +380,10 @@ __global__ __launch_bounds__(kWideThreads) void k_wide_scatter(
     const int s = it.s, sz = it.sz;
     const int p0 = c * kWideChunk;
     if (p0 >= sz) return;
-    int abase = 0, tbase = 0, K = 0, Tn = 0;
-    for (int q = 0; q < wg.chunks; q++) {
-        if (q * kWideChunk >= sz) break;
-        const int* r = wg.at(merge, q);
-        if (q < c) {
-            abase += r[kWNa];
-            tbase += r[kWNt];
-        }
-        K += r[kWNa];
-        Tn += r[kWNt];
-    }
+    const int nq = wide_chunks_in(wg, sz);
+    const WideSums sa = wide_field_sums(wg, merge, nq, c, kWNa);
+    const WideSums st = wide_field_sums(wg, merge, nq, c, kWNt);
+    const int abase = sa.below, tbase = st.below, K = sa.total, Tn = st.total;
     constexpr int per = kWideChunk / kWideThreads;
     const int pa = p0 + (int)threadIdx.x * per;
     int f[per], na = 0, nt = 0;

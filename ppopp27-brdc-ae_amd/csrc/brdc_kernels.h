@@ -1004,6 +1004,15 @@ __device__ __forceinline__ CPairs as_const(const double2* q) {
 }
 __device__ __forceinline__ CDoubles as_const(const double* q) { return CDoubles{(cdbl)q}; }
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// K of merges m0 .. m0+63 of a level, merge m0 + l in lane l (0 past the
+// last merge): one load for 64 merges, where a loop over merges reading one
+// count at a time would wait a memory round trip per merge
+__device__ __forceinline__ int merge_counts(const int* counts, int m0, int merges) {
+    const int m = m0 + (int)(threadIdx.x & (kWave - 1));
+    return (m < merges) ? counts[kCountStride * m] : 0;
+}
+__device__ __forceinline__ int lane_value(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 constexpr int kMaxWaves = 8;  // 512 threads: the root solver needs ~180 VGPRs per lane
 
 __host__ __device__ inline int waves_for(int K, int Wmax) {
@@ -1766,8 +1775,10 @@ __global__ __launch_bounds__(kWave * W) void k_secular_mid(Geo g, int merges, in
     const int wave = uniform(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     int t = blockIdx.x;  // this workgroup's next tile
     int tbase = 0;       // tiles of the streamed merges before `merge`
+    int Kl = 0;          // counts of merges 64c .. 64c+63, c = merge / 64
     for (int merge = 0; merge < merges; merge++) {
-        const int K = uniform(counts[kCountStride * merge]);
+        if ((merge & (kWave - 1)) == 0) Kl = merge_counts(counts, merge, merges);
+        const int K = lane_value(Kl, merge & (kWave - 1));
         if (K < kstream) continue;
         const int nt = (K + kWave - 1) / kWave;
         for (; t < tbase + nt; t += gridDim.x) {
@@ -1892,10 +1903,14 @@ void k_secular_stream(
     // interval meets its own, taking roots there until the merge's counter
     // runs out.
     long long wsum = 0;
-    for (int m = 0; m < merges; m++) {
-        const long long K = uniform(counts[kCountStride * m]);
-        if (K >= kstream) wsum += K * K;
+    for (int m0 = 0; m0 < merges; m0 += kWave) {
+        const long long Kl = merge_counts(counts, m0, merges);
+        long long wl = (Kl >= kstream) ? Kl * Kl : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) wl += __shfl_xor(wl, o, kWave);
+        wsum += ((long long)uniform((int)(wl >> 32)) << 32) | (unsigned)uniform((int)wl);
     }
+    if (wsum == 0) return;  // no merge of the level is streamed
     // workgroup b takes interval v = 2 (b mod h) + b div h, h = ceil(G/2): the
     // SIMDs favour older waves and a CU's two workgroups are either b and
     // b + h (the younger) or neighbours, so every merge gets as many of the
@@ -2035,7 +2050,7 @@ void k_secular_stream(
                     const RootState& r = st.out;
                     if (c == 0) {
                         rlam[s + i] = r.lambda;
-                        rs[s + i] = make_double2(dz[r.org].x, r.tau);
+                        rs[s + i] = make_double2(st.dorg, r.tau);  // dorg = d_org
                         rn1[s + i] = r.near1;
                         rn2[s + i] = r.near2;
                         if (r.info) {
