@@ -331,15 +331,33 @@ __device__ __forceinline__ bool leaf_ql(int s, int m, int n, double scale,
 #define se_(k) ve[(k) * ls]
 #define s0_(k) v0[(k) * ls]
 #define s1_(k) v1[(k) * ls]
-    for (int k = 0; k < m; k++) {
-        double dk = d_in[s + k] * scale;
-        if (k == 0 && s > 0) dk -= fabs(ew[s - 1]);
-        if (k == m - 1 && s + m < n) dk -= fabs(ew[s + m - 1]);
+    // the leaf's entries, four rows at a time with their loads in flight
+    // together
+    const double ecut_lo = (s > 0) ? fabs(ew[s - 1]) : 0.0;
+    const double ecut_hi = (s + m < n) ? fabs(ew[s + m - 1]) : 0.0;
+    auto put = [&](int k, double dv, double ev) {
+        // the shift rounds after the scaling, as a separate subtraction
+#pragma clang fp contract(off)
+        double dk = dv * scale;
+        if (k == 0 && s > 0) dk -= ecut_lo;
+        if (k == m - 1 && s + m < n) dk -= ecut_hi;
         sd_(k) = dk;
-        se_(k) = (k < m - 1) ? ew[s + k] : 0.0;
+        se_(k) = (k < m - 1) ? ev : 0.0;
         s0_(k) = (k == 0) ? 1.0 : 0.0;
         s1_(k) = (k == m - 1) ? 1.0 : 0.0;
+    };
+    int k0 = 0;
+    for (; k0 + 4 <= m; k0 += 4) {
+        double dv[4], ev[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            dv[u] = d_in[s + k0 + u];
+            ev[u] = (k0 + u < m - 1) ? ew[s + k0 + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) put(k0 + u, dv[u], ev[u]);
     }
+    for (; k0 < m; k0++) put(k0, d_in[s + k0], (k0 < m - 1) ? ew[s + k0] : 0.0);
     bool failed = false;
     const int maxit = 30 * m;
     int iter = 0;
@@ -517,23 +535,50 @@ __global__ __launch_bounds__(LP) void k_leaf(
 // first, as DLAMRG does) and writes z = [last rows of left; sign(rho) * first
 // rows of right] / sqrt(2) with the parent's two boundary rows
 // q1 = [blo_left, 0], q2 = [0, bhi_right].
-__device__ __forceinline__ int count_less(const double* a, int len, double x) {
-    int lo = 0, hi = len;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1;
-        else hi = mid;
+// Entries of the ascending a[0..len) below x (kLE: at most x).  An 8-way
+// search: each round reads seven independent probes and keeps the eighth of
+// [lo, hi) they bracket, so a search of n entries waits about log8(n) + 1
+// memory round trips instead of the log2(n) of a bisection.
+template <bool kLE>
+__device__ __forceinline__ int count_before(const double* a, int len, double x) {
+    int lo = 0, hi = len;  // a[i] passes for i < lo and fails for i >= hi
+    while (hi - lo > 8) {
+        const int span = hi - lo;
+        int p[7];
+        double v[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            p[k] = lo + (int)(((long long)span * (k + 1)) >> 3);
+            v[k] = a[p[k]];
+        }
+        // the passing probes are a prefix of the seven: lo moves past the
+        // last of them, hi to the first failing one (selects, no indexing)
+        int nlo = lo, nhi = hi;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            const bool pass = kLE ? (v[k] <= x) : (v[k] < x);
+            if (pass) nlo = p[k] + 1;
+            if (!pass && nhi == hi) nhi = p[k];
+        }
+        lo = nlo;
+        hi = nhi;
     }
-    return lo;
+    int c = 0;  // at most eight entries left, read together
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int i = lo + k;
+        if (i < hi) {
+            const double v = a[i];
+            c += (kLE ? (v <= x) : (v < x)) ? 1 : 0;
+        }
+    }
+    return lo + c;
+}
+__device__ __forceinline__ int count_less(const double* a, int len, double x) {
+    return count_before<false>(a, len, x);
 }
 __device__ __forceinline__ int count_less_equal(const double* a, int len, double x) {
-    int lo = 0, hi = len;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] <= x) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+    return count_before<true>(a, len, x);
 }
 
 // Element idx < sz of the merge at rows [s, s + sz), rho the off-diagonal at

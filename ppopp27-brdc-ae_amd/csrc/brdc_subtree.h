@@ -92,83 +92,26 @@ __device__ __forceinline__ int sub_locate(int S, int r, int e, int* ms, int* msz
     return idx;
 }
 
-// Product-form entry of pole i of a K-pole merge (K >= 3) at LDS offset ms:
-// w_i = sign(z_i) sqrt(-prod_j delta_j(i) / prod_{j != i} (d_i - d_j)), the
-// pole-adjacent deltas from the roots' near1/near2 (as k_wprod).
-__device__ __forceinline__ double2 sub_wprod(const SubtreeLds& L, int ms, int K, int i) {
-    const double di = L.ad[ms + i];
-    double num = 1.0, den = 1.0;
-    int ex = 0;
-    auto renorm = [&]() {
-        const int en = __builtin_amdgcn_frexp_exp(num), ed = __builtin_amdgcn_frexp_exp(den);
-        num = __builtin_amdgcn_ldexp(num, -en);
-        den = __builtin_amdgcn_ldexp(den, -ed);
-        ex += en - ed;
-    };
-    for (int j = 0; j < K; j++) {
-        const double2 r = L.rs[ms + j];
-        double dl = (di - r.x) - r.y;
-        if (j < K - 1) {
-            if (i == j) dl = L.rn1[ms + j];
-            else if (i == j + 1) dl = L.rn2[ms + j];
-        } else {  // the last root: near1 at pole K-2, near2 at pole K-1
-            if (i == K - 2) dl = L.rn1[ms + j];
-            else if (i == K - 1) dl = L.rn2[ms + j];
-        }
-        num *= dl;
-        den *= (j == i) ? 1.0 : di - L.ad[ms + j];
-        if ((j & 3) == 3) renorm();
+// Boundary rows of root j of a merge of K <= 2 poles at LDS offset ms: the
+// secular eigenvector (near1, near2) applied directly (dlaed9_br.f:146-172).
+__device__ __forceinline__ void sub_rows_direct(const SubtreeLds& L, int ms, int K, int j,
+                                                double* q1o, double* q2o) {
+    const double v0 = L.rn1[ms + j], v1 = L.rn2[ms + j];
+    const double2 q0 = L.aq[ms];
+    double a1 = q0.x * v0, a2 = q0.y * v0;
+    if (K == 2) {
+        const double2 q1 = L.aq[ms + 1];
+        a1 += q1.x * v1;
+        a2 += q1.y * v1;
     }
-    renorm();
-    const double prod = __builtin_amdgcn_ldexp(num / den, ex);
-    return make_double2(di, copysign(sqrt(-prod), L.az[ms + i]));
-}
-
-// Boundary rows of root j of a merge at LDS offset ms (as k_rows).
-__device__ __forceinline__ void sub_rows(const SubtreeLds& L, int ms, int K, int j, double* q1o,
-                                         double* q2o) {
-    if (K <= 2) {  // eigenvector returned directly (dlaed9_br.f:146-172)
-        const double v0 = L.rn1[ms + j], v1 = L.rn2[ms + j];
-        const double2 q0 = L.aq[ms];
-        double a1 = q0.x * v0, a2 = q0.y * v0;
-        if (K == 2) {
-            const double2 q1 = L.aq[ms + 1];
-            a1 += q1.x * v1;
-            a2 += q1.y * v1;
-        }
-        *q1o = a1;
-        *q2o = a2;
-        return;
-    }
-    const double2 r = L.rs[ms + j];
-    const double dorg = r.x, tau = r.y, nr1 = L.rn1[ms + j], nr2 = L.rn2[ms + j];
-    const int a = (j == K - 1) ? K - 2 : j;
-    double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
-    for (int i = 0; i < K; i++) {
-        const double2 p = L.pw[ms + i];
-        double dl = (p.x - dorg) - tau;
-        if (i == a) dl = nr1;
-        else if (i == a + 1) dl = nr2;
-        const double x = p.y * rcp_stream(dl);
-        const double2 q = L.aq[ms + i];
-        temp = fma(x, x, temp);
-        acc1 = fma(q.x, x, acc1);
-        acc2 = fma(q.y, x, acc2);
-    }
-    const double nrm = sqrt(temp);
-    *q1o = acc1 / nrm;
-    *q2o = acc2 / nrm;
+    *q1o = a1;
+    *q2o = a2;
 }
 
 // The secular roots of a level (at most kSubRows) go two lanes each, dealt
 // round the block's waves, and small merges deflate one per wave; a SIMD
 // holds two such waves, which hide part of each other's dependent FP64
 // chains.
-// Item (a pole or root) of thread tid in the phases that carry FP64 work:
-// consecutive items go to consecutive waves.
-__device__ __forceinline__ int sub_item_of(int tid) {
-    return (tid & (kWave - 1)) * kSubWaves + (tid >> 6);
-}
 
 // One block per subtree root (blockIdx.x).  out_final: the eigenvalues when
 // the subtree is its whole matrix; od/olo/ohi: the subtree root's node state
@@ -332,21 +275,64 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(
             rec[6] = L.nfail[tid];
         }
         if (keep_rows) {
-            // poles and roots alternate between the waves
-            const int j = sub_item_of(tid);
-            int m = 0;
-            while (m + 1 < nm && L.roff[m + 1] <= j) m++;
-            const bool has = j < total;
-            const int ms = L.mst[m], K = L.K[m], i = j - L.roff[m];
-            // ---- product form, one thread per pole
-            if (has && K >= 3) L.pw[ms + i] = sub_wprod(L, ms, K, i);
+            // Product form and boundary rows: Lp lanes per pole (root), the
+            // most (up to 16) that leave every item of the level a lane
+            // group; lane c takes its share [K c / Lp, K (c+1) / Lp) of the
+            // merge's roots (poles), and the shares meet in a butterfly (as
+            // the level kernels' small merges)
+            int Lp = 1;
+            while (Lp < 16 && 2 * Lp * total <= kSubThreads) Lp <<= 1;
+            const int c = tid & (Lp - 1), per = kSubThreads / Lp;
+            // the same trip count in every lane: items past the last are
+            // computed on the last item and not stored
+            auto item = [&](int base, int& ms, int& K, int& i) {
+                const int j = min(base + tid / Lp, total - 1);
+                int m = 0;
+                while (m + 1 < nm && L.roff[m + 1] <= j) m++;
+                ms = L.mst[m];
+                K = L.K[m];
+                i = j - L.roff[m];
+            };
+            // ---- product form
+            for (int base = 0; base < total; base += per) {
+                int ms, K, i;
+                item(base, ms, K, i);
+                const int ra = (K * c) / Lp, rb = (K * (c + 1)) / Lp;
+                double share = -1.0;
+                if (K >= 3)
+                    share = wprod_share(RootsLds{L.rs + ms, L.ad + ms}, L.rn1 + ms, L.rn2 + ms, i,
+                                        L.ad[ms + i], i & ~(kGroup - 1), ra, rb, K);
+                for (int o = 1; o < Lp; o <<= 1) share *= __shfl_xor(share, o);
+                if (c == 0 && base + tid / Lp < total && K >= 3)
+                    L.pw[ms + i] = make_double2(L.ad[ms + i], copysign(sqrt(-share), L.az[ms + i]));
+            }
             __syncthreads();
-            // ---- boundary rows, one thread per root
-            if (has) {
-                double q1, q2;
-                sub_rows(L, ms, K, i, &q1, &q2);
-                L.qo1[ms + i] = q1;
-                L.qo2[ms + i] = q2;
+            // ---- boundary rows
+            for (int base = 0; base < total; base += per) {
+                int ms, K, j;
+                item(base, ms, K, j);
+                const int ia = (K * c) / Lp, ib = (K * (c + 1)) / Lp;
+                double temp = 0.0, acc1 = 0.0, acc2 = 0.0;
+                if (K >= 3)
+                    rows_share(PolesLds{L.pw + ms, L.aq + ms}, j, L.rs[ms + j], L.rn1[ms + j],
+                               L.rn2[ms + j], j & ~(kGroup - 1), ia, ib, K, temp, acc1, acc2);
+                for (int o = 1; o < Lp; o <<= 1) {
+                    temp += __shfl_xor(temp, o);
+                    acc1 += __shfl_xor(acc1, o);
+                    acc2 += __shfl_xor(acc2, o);
+                }
+                if (c == 0 && base + tid / Lp < total) {
+                    double q1, q2;
+                    if (K >= 3) {
+                        const double nrm = sqrt(temp);
+                        q1 = acc1 / nrm;
+                        q2 = acc2 / nrm;
+                    } else {
+                        sub_rows_direct(L, ms, K, j, &q1, &q2);
+                    }
+                    L.qo1[ms + j] = q1;
+                    L.qo2[ms + j] = q2;
+                }
             }
             __syncthreads();
         }
