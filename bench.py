@@ -345,6 +345,7 @@ def bench_differentiated(steps, warmup, compare=True):
         return w
 
     def timed(fn, reps):
+        fn()  # untimed, directly before (clock settling)
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -569,7 +570,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=49152)
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -657,11 +658,10 @@ def main():
                 w_dev[g].copy_(d_dev[g])
                 solver.solve(w_dev[g], e_dev[g], stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    # correctness of the results the timed region reproduces
+    # correctness of the results the timed region reproduces (before the
+    # warm-up, so that the warm-up steps run directly before the timed ones:
+    # the GPU's clock drops within moments of idling, and the host work here
+    # would otherwise leave the timed region starting on a lowered clock)
     step()
     torch.cuda.synchronize()
     infos = {}
@@ -676,6 +676,8 @@ def main():
             infos[g] = int(solver.info.item())
             outs[g] = w_dev[g].cpu().numpy()
 
+    for _ in range(args.warmup):
+        step()
     # timed region: exactly `steps` steps between two synchronisations
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -694,6 +696,8 @@ def main():
     # per-generator ms (graph replay and eager launches) and per-kernel
     # breakdown (separate passes over the same work)
     def timed(fn, reps):
+        for _ in range(2):  # untimed, directly before (clock settling)
+            fn()
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(stream)
