@@ -2198,11 +2198,15 @@ __device__ __forceinline__ double wprod_share(const R& roots, const double* n1, 
     // roots whose near1/near2 apply to a pole of this group: j in [g0-1, g0+63]
     // (near1 at pole j, near2 at pole j+1) and the last root (poles K-2, K-1)
     const int wa = max(j0, g0 - 1), wb = min(j1, g0 + kGroup);
+    // the near deltas this pole can take in the loop: near1 of root i and
+    // near2 of root i-1 (read once, not per pair)
+    const double a1 = (i < K - 1) ? n1[i] : 0.0;
+    const double a2 = (i >= 1) ? n2[i - 1] : 0.0;
     auto term_to = [&](double& nu, double& de, double2 r, double dj, int j, bool check) {
         double dl = (di - r.x) - r.y;
         if (check) {  // j == i and the pole-adjacent roots lie in the window only
-            if (i == j) dl = n1[j];
-            else if (i == j + 1) dl = n2[j];
+            if (i == j) dl = a1;
+            else if (i == j + 1) dl = a2;
             nu *= dl;
             de *= (j == i) ? 1.0 : di - dj;
         } else {
