@@ -80,8 +80,18 @@ def _store(path, cache):
         os.replace(tmp, path)
 
 
-def _search(time_of, leaf_candidates, knobs: bool):
-    """Coordinate search: time_of(config) -> ms for a dict of options."""
+# A candidate replaces the configuration kept so far only when it is faster by
+# more than this fraction: replays of one configuration spread by about half a
+# per cent, so smaller wins are noise, and taking them made the chosen launch
+# shape (and with it the per-kernel launch mix) change from run to run.
+MARGIN = 0.01
+
+
+def _search(time_of, leaf_candidates, knobs: bool, margin: float = MARGIN):
+    """Coordinate search: time_of(config) -> ms for a dict of options.  The
+    first value of each candidate list is the starting point; a later value
+    is taken only if it beats the kept configuration by more than `margin`
+    (a fraction of the kept configuration's time)."""
     timings = {}
 
     def run(cfg):
@@ -90,18 +100,21 @@ def _search(time_of, leaf_candidates, knobs: bool):
             timings[key] = time_of(cfg)
         return timings[key]
 
+    def better(cfg, kept):
+        return run(cfg) < run(kept) * (1.0 - margin)
+
     best = {"leaf_max": int(leaf_candidates[0])}
     if knobs:
         best.update({k: v[0] for k, v in KNOB_CANDIDATES.items()})
     for leaf in leaf_candidates:
         cfg = dict(best, leaf_max=int(leaf))
-        if run(cfg) < run(best):
+        if better(cfg, best):
             best = cfg
     if knobs:
         for name, values in KNOB_CANDIDATES.items():
             for v in values:
                 cfg = dict(best, **{name: int(v)})
-                if run(cfg) < run(best):
+                if better(cfg, best):
                     best = cfg
     return best, timings
 

@@ -160,3 +160,27 @@ def test_harness_argument_errors():
         harness.main([f"--input={DEBIN}", "--run-self=0"], out=io.StringIO())  # dense path
     with pytest.raises(SystemExit):
         harness.main(["--data=nosuchmode", "--run-self=0"], out=io.StringIO())
+
+
+def test_autotune_search_keeps_the_start_unless_a_candidate_wins_by_the_margin():
+    """The coordinate search of brdc.autotune: a candidate within the noise
+    margin of the kept configuration does not replace it; a clear win does,
+    and later knobs are searched from the new configuration."""
+    from brdc import autotune
+
+    def time_of(cfg):
+        t = 10.0
+        t -= {8: 0.0, 12: 0.05, 16: 0.5}[cfg["leaf_max"]]      # 16 wins by 5 %
+        t -= 0.05 if cfg["stream_min_k"] == 1024 else 0.0    # 0.5 %: noise
+        t -= 0.3 if cfg["wide_max_waves"] == 4 else 0.0      # 3 %: a win
+        return t
+
+    best, timings = autotune._search(time_of, (8, 12, 16), knobs=True)
+    assert best["leaf_max"] == 16
+    assert best["stream_min_k"] == autotune.KNOB_CANDIDATES["stream_min_k"][0]
+    assert best["wide_max_waves"] == 4
+    assert best["secular_lanes_log2"] == autotune.KNOB_CANDIDATES["secular_lanes_log2"][0]
+    # every candidate was timed once, from the configuration kept at its turn
+    assert len(timings) == 3 + sum(len(v) - 1 for v in autotune.KNOB_CANDIDATES.values())
+    best0, _ = autotune._search(time_of, (8, 12, 16), knobs=True, margin=0.0)
+    assert best0["stream_min_k"] == 1024
